@@ -1,0 +1,135 @@
+/*
+ * rtm_hip.h -- C ABI of the MI355X (gfx950) RTM library, librtm_hip.so.
+ *
+ * The reference (caixh90/RTM_GPU) has no library boundary: its 17 __global__
+ * kernels are launched from main() in the same translation unit, with raw
+ * device pointers and scalars as arguments (kernel.cu:18-517, launched at
+ * kernel.cu:808-929).  These entry points replace those launches and the
+ * device-memory handling around them (kernel.cu:758-786, 805-806, 822-825,
+ * 878-885, 909, 932-933).  The host driver that calls them keeps the
+ * reference's shot loop (kernel.cu:791-991).
+ *
+ * Conventions
+ *  - All host arrays are in the reference layout: row-major [z][x] with x
+ *    contiguous, NZ x NX padded grid (kernel.cu:611-612).
+ *  - Indices are 0-based padded grid indices, i.e. after the 1-based to
+ *    padded conversion of kernel.cu:607-610 and kernel.cu:794-795.
+ *  - Every entry point returns 0 on success or a negative RTM_E* code; the
+ *    message is available from rtm_last_error().  No C++ exception crosses
+ *    the ABI.  A context is used by one host thread at a time; distinct
+ *    contexts (one per pthread) may run concurrently.
+ *  - The library owns all device memory of a context; callers own host
+ *    buffers.  Device work is ordered on the context's own HIP stream.
+ */
+#ifndef RTM_HIP_H
+#define RTM_HIP_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RTM_OK = 0,
+  RTM_E_ARG = -1,     /* bad argument or call order */
+  RTM_E_HIP = -2,     /* HIP runtime error */
+  RTM_E_NOMEM = -3,   /* device allocation failed */
+  RTM_E_STATE = -4    /* entry point called in the wrong state */
+};
+
+enum { RTM_STRIP_F32 = 0, RTM_STRIP_F16 = 1, RTM_STRIP_BF16 = 2 };
+
+enum {
+  RTM_FIELD_U = 0,      /* forward / reconstructed source field, newest step */
+  RTM_FIELD_U_PREV = 1, /* same field, previous step */
+  RTM_FIELD_Q = 2,      /* receiver (back-propagated) field, newest step */
+  RTM_FIELD_I1 = 3,     /* cross-correlation image (Drel1) */
+  RTM_FIELD_I2 = 4,     /* source illumination (Drel2) */
+  RTM_FIELD_SU = 5,     /* running sum of u (DBWADD) */
+  RTM_FIELD_SQ = 6      /* running sum of q (DFWADD) */
+};
+
+/* Kernel kinds for rtm_kernel_time(). */
+enum {
+  RTM_K_FWD_INTERIOR = 0, RTM_K_FWD_FRAME = 1, RTM_K_SRC_INTERIOR = 2, RTM_K_STRIP_RESTORE = 3,
+  RTM_K_RCV_INTERIOR = 4, RTM_K_RCV_FRAME = 5, RTM_K_NKINDS = 6
+};
+
+typedef struct rtm_params {
+  int nz, nx;            /* padded grid NZ, NX (kernel.cu:611-612) */
+  int n2;                /* hybrid ABC width N2 */
+  int nfdmax;            /* maximum operator half-length; also strip depth */
+  int nt, nt2;           /* NT and the wavelet gate NT2 (kernel.cu:613,624) */
+  int ilste;             /* 0 adaptive LS coefficients, 1 Taylor (kernel.cu:744-753) */
+  int icompen;           /* 1 compensated imaging (Rel_Compen), 0 Rel_NonCompen */
+  int s_l, s_r, s_z, ds, n; /* receiver line (kernel.cu:607-610) */
+  float tao2, h2, taoh, taoh2, hzx2_1; /* step scalars (kernel.cu:614-617,625-626) */
+  int max_shots;         /* shots per batch this context can hold */
+  int strip_format;      /* RTM_STRIP_F32 (reference), RTM_STRIP_F16, RTM_STRIP_BF16 */
+  int record;            /* 1: allocate receiver records for forward modelling */
+  int zseg;              /* rows per wave in the interior kernels (0 = auto) */
+} rtm_params;
+
+typedef struct rtm_ctx rtm_ctx;
+
+/* Allocate a context on the current HIP device (replaces kernel.cu:758-779). */
+int rtm_create(const rtm_params* p, rtm_ctx** out);
+int rtm_destroy(rtm_ctx* ctx);
+const char* rtm_last_error(const rtm_ctx* ctx);
+/* Device bytes held by the context. */
+size_t rtm_device_bytes(const rtm_ctx* ctx);
+
+/* Upload the model (replaces kernel.cu:781-786).  v, r1: [nz*nx] as built by
+ * velocity() (GPU_velocity_real.cpp:6-118); w: [n2+1] (kernel.cu:687-691);
+ * c[nc], index[nvel+1]: funMandC's table (LSMOrCon_rec_2D.cpp:22-73), or
+ * c[nfdmax+1] from order() with index = NULL when ilste = 1; vmin, dv as
+ * snapped at kernel.cu:715-720. */
+int rtm_set_model(rtm_ctx* ctx, const float* v, const float* r1, const float* w, const float* c,
+                  int nc, const int* index, int nvel, float vmin, float dv);
+/* Wavelet table wav[j] = f(j*tao), j = 0..nt (kernel.cu:812, 889, 1261). */
+int rtm_set_wavelet(rtm_ctx* ctx, const float* wav);
+/* Sources of the batch: shot i injects at (r_u[i], r_x[i]) (kernel.cu:794-795). */
+int rtm_set_shots(rtm_ctx* ctx, int nshot, const int* r_u, const int* r_x);
+/* Observed gathers of the batch, seis[shot][n][nt] as read at kernel.cu:827-856.
+ * Uploaded once per batch (the reference copies one time slice per step,
+ * kernel.cu:905-909). */
+int rtm_set_gathers(rtm_ctx* ctx, const float* seis);
+
+/* Forward pass (kernel.cu:798-821): begin sets u(0)=0, u(1)=f(0)/2 at the
+ * source and strip slots 0,1; advance runs up to nsteps of k = 2..nt-1.
+ * save_strips = 0 skips the boundary strips (pure modelling). */
+int rtm_forward_begin(rtm_ctx* ctx, int save_strips);
+int rtm_forward_advance(rtm_ctx* ctx, int nsteps);
+/* Backward pass with imaging (kernel.cu:822-931). */
+int rtm_backward_begin(rtm_ctx* ctx);
+int rtm_backward_advance(rtm_ctx* ctx, int nsteps);
+/* Forward then backward for the whole batch. */
+int rtm_migrate(rtm_ctx* ctx);
+/* Block until the context's stream is idle. */
+int rtm_sync(rtm_ctx* ctx);
+
+/* Read back (debug / parity): a field of one shot into out[nz*nx]. */
+int rtm_read_field(rtm_ctx* ctx, int shot, int which, float* out);
+/* Receiver records of the batch, out[shot][n][nt] (forward modelling). */
+int rtm_read_records(rtm_ctx* ctx, float* out);
+/* Strip slot k of one shot, decoded to float, layout lf[mnz][nfd],
+ * rt[mnz][nfd], up[nfd][mnx], dw[nfd][mnx] (Dlf/Drt/Dup/Ddw, kernel.cu:199-205). */
+int rtm_read_strip(rtm_ctx* ctx, int shot, int k, float* out);
+
+/* Per-shot image post (kernel.cu:935-990): up = -lap(I1)*v^2/vmax^2,
+ * down = I2 + whitecoe*max|I2|, both interior [mnz][mnx] row-major, for every
+ * shot of the batch, written to up[shot][..], down[shot][..] when non-NULL,
+ * and added in shot order to the context's stack (kernel.cu:1000-1040). */
+int rtm_image_post(rtm_ctx* ctx, float vmax, float whitecoe, float* up, float* down);
+/* Stack result (kernel.cu:1042-1059): img = sum(up)/nrec, illum = sum(down)/nrec,
+ * img /= illum when inorm = 1.  Interior [mnz][mnx].  Resets the stack. */
+int rtm_stack_finish(rtm_ctx* ctx, int nrec, int inorm, float* img, float* illum);
+
+/* Kernel timing with HIP events on the context stream (0 = off). */
+int rtm_set_timing(rtm_ctx* ctx, int on);
+/* Accumulated device time (ms) and launch count of one kernel kind. */
+int rtm_kernel_time(rtm_ctx* ctx, int kind, double* total_ms, long long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,693 @@
+// rtm_capi.hip -- the C ABI of librtm_hip.so (declared in include/rtm_hip.h).
+//
+// Owns the device memory of a context and sequences the kernels of
+// rtm_kernels.hip on the context's stream.  Replaces the device-side part of
+// the reference's main(): allocation and uploads (kernel.cu:758-786), the
+// forward time loop (kernel.cu:798-825), the backward loop with imaging
+// (kernel.cu:859-933).  Deliver/Deliver_EFF copies are replaced by rotating
+// buffer indices; the per-step host-to-device data copy (kernel.cu:905-909)
+// by one upload of the whole gather per batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/rtm_hip.h"
+#include "rtm_kernels.h"
+
+using namespace rtm;
+
+struct EventPair {
+  hipEvent_t a, b;
+  int kind;
+};
+
+struct rtm_ctx {
+  rtm_params p{};
+  Grid g{};
+  Model m{};
+  Rcv r{};
+  Strips st{};
+  int nfd = 0;           // compile-time stencil half-length used
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  size_t bytes = 0;
+  std::vector<void*> allocs;
+  // device buffers
+  float *d_v = nullptr, *d_r1 = nullptr, *d_ctab = nullptr, *d_w = nullptr;
+  float* F[3] = {nullptr, nullptr, nullptr};
+  float* Q[3] = {nullptr, nullptr, nullptr};
+  float *SU = nullptr, *SQ = nullptr, *I1 = nullptr, *I2 = nullptr;
+  float *d_data = nullptr, *d_rec = nullptr;
+  int2* d_src = nullptr;
+  int4* d_tiles = nullptr;
+  int ntiles = 0;
+  float *d_up = nullptr, *d_down = nullptr, *d_s1 = nullptr, *d_s2 = nullptr;
+  unsigned* d_imax = nullptr;
+  // host-side state
+  std::vector<float> wav;
+  std::vector<int2> src;
+  int nshot = 0;
+  int f0 = 0, f1 = 1, f2 = 2, q0 = 0, q1 = 1, q2 = 2;
+  int kf = -1, kb = -2;  // next forward / backward step; -1/-2 = not begun
+  bool model_ok = false, wav_ok = false, shots_ok = false, data_ok = false, save_strips = true;
+  int zs = 32, nxt = 1, nblk = 1;
+  // timing
+  bool timing = false;
+  std::vector<EventPair> pending;
+  std::vector<hipEvent_t> pool;
+  double t_ms[RTM_K_NKINDS] = {0};
+  long long t_n[RTM_K_NKINDS] = {0};
+};
+
+namespace {
+
+int fail(rtm_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                     \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((c), RTM_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),     \
+                  __FILE__, __LINE__);                                                      \
+  } while (0)
+
+int dalloc(rtm_ctx* c, void** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 16;
+  hipError_t e = hipMalloc(p, n);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, RTM_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", n, hipGetErrorString(e));
+  }
+  c->allocs.push_back(*p);
+  c->bytes += n;
+  return RTM_OK;
+}
+template <typename T>
+int dalloc_t(rtm_ctx* c, T** p, size_t count) {
+  return dalloc(c, reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+// IEEE binary16 -> float on the host.
+float half_to_float(unsigned short h) {
+  const unsigned s = (h >> 15) & 1u, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  float f;
+  if (e == 0) f = ldexpf((float)m, -24);
+  else if (e == 31) f = m ? NAN : INFINITY;
+  else f = ldexpf((float)(m | 0x400u), (int)e - 25);
+  return s ? -f : f;
+}
+
+// ---- timing -------------------------------------------------------------------
+hipEvent_t get_event(rtm_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+int drain_timing(rtm_ctx* c) {
+  if (c->pending.empty()) return RTM_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto& ep : c->pending) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, ep.a, ep.b));
+    c->t_ms[ep.kind] += ms;
+    c->t_n[ep.kind] += 1;
+    c->pool.push_back(ep.a);
+    c->pool.push_back(ep.b);
+  }
+  c->pending.clear();
+  return RTM_OK;
+}
+struct Timed {
+  rtm_ctx* c;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(rtm_ctx* c_, int k) : c(c_), kind(k) {
+    if (c->timing) {
+      a = get_event(c);
+      b = get_event(c);
+      if (a && b) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~Timed() {
+    if (c->timing && a && b) {
+      (void)hipEventRecord(b, c->stream);
+      c->pending.push_back({a, b, kind});
+      if (c->pending.size() >= 4096) drain_timing(c);
+    }
+  }
+};
+
+// ---- launch helpers -----------------------------------------------------------
+void fill_int_args(rtm_ctx* c, IntArgs& a) {
+  a.g = c->g;
+  a.m = c->m;
+  a.r = c->r;
+  a.src = c->d_src;
+  a.nt = c->p.nt;
+  a.nxt = c->nxt;
+  a.zs = c->zs;
+  a.f64comb = c->p.ilste != 0;
+  a.rec = nullptr;
+  a.data = c->d_data;
+  a.uk = nullptr;
+  a.su = c->SU; a.sq = c->SQ; a.i1 = c->I1; a.i2 = c->I2;
+  a.icompen = c->p.icompen;
+}
+void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
+  a.g = c->g;
+  a.m = c->m;
+  a.r = c->r;
+  a.tiles = c->d_tiles;
+  a.src = c->d_src;
+  a.nt = c->p.nt;
+  a.f64comb = c->p.ilste != 0;
+  a.rec = nullptr;
+  a.data = c->d_data;
+  a.st = c->st;
+  a.save_strips = 0;
+}
+
+int forward_step(rtm_ctx* c, int k) {
+  const float wv = (k < c->p.nt2) ? c->wav[k - 1] : 0.0f;  // kernel.cu:812-813
+  const bool quant = c->save_strips && c->st.fmt != RTM_STRIP_F32;
+  if (quant && k >= 3) HIPCHK(c, launch_strip_finish(c->g, c->st, k - 1, c->nshot, c->stream));
+  {
+    IntArgs a;
+    fill_int_args(c, a);
+    a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
+    a.wav = wv; a.k = k;
+    a.rec = c->p.record ? c->d_rec : nullptr;
+    Timed t(c, RTM_K_FWD_INTERIOR);
+    HIPCHK(c, launch_interior(EPI_FWD, c->nfd, a, c->nshot, c->nblk, c->stream));
+  }
+  {
+    FrameArgs a;
+    fill_frame_args(c, a);
+    a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
+    a.wav = wv; a.k = k;
+    a.rec = c->p.record ? c->d_rec : nullptr;
+    a.save_strips = c->save_strips ? 1 : 0;
+    Timed t(c, RTM_K_FWD_FRAME);
+    HIPCHK(c, launch_frame(FR_FWD, c->nfd, a, c->nshot, c->ntiles, c->stream));
+  }
+  if (quant && k == c->p.nt - 1) HIPCHK(c, launch_strip_finish(c->g, c->st, k, c->nshot, c->stream));
+  int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t;  // Deliver (kernel.cu:820)
+  return RTM_OK;
+}
+
+int backward_step(rtm_ctx* c, int k) {
+  const float wv = (k < c->p.nt2) ? c->wav[k + 1] : 0.0f;  // kernel.cu:889-890
+  {  // BKAdd_EFF (kernel.cu:896): u(k) on the interior
+    IntArgs a;
+    fill_int_args(c, a);
+    a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
+    a.wav = wv; a.k = k;
+    Timed t(c, RTM_K_SRC_INTERIOR);
+    HIPCHK(c, launch_interior(EPI_SRC, c->nfd, a, c->nshot, c->nblk, c->stream));
+  }
+  {  // strips of slot k into the rings of u(k): BKEqual of step k-1 (kernel.cu:892)
+    Timed t(c, RTM_K_STRIP_RESTORE);
+    HIPCHK(c, launch_strip_restore(c->g, c->F[c->f2], c->st, k, c->nshot, c->stream));
+  }
+  { int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t; }  // Deliver_EFF (kernel.cu:903)
+  {  // BKAdd (kernel.cu:913) + Rel_* (kernel.cu:923-930)
+    IntArgs a;
+    fill_int_args(c, a);
+    a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
+    a.uk = c->F[c->f1];
+    a.wav = 0.f; a.k = k;
+    Timed t(c, RTM_K_RCV_INTERIOR);
+    HIPCHK(c, launch_interior(EPI_RCV, c->nfd, a, c->nshot, c->nblk, c->stream));
+  }
+  {  // BKAdd on the frame + BKHybrid1/2 (kernel.cu:913-920)
+    FrameArgs a;
+    fill_frame_args(c, a);
+    a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
+    a.wav = 0.f; a.k = k;
+    Timed t(c, RTM_K_RCV_FRAME);
+    HIPCHK(c, launch_frame(FR_RCV, c->nfd, a, c->nshot, c->ntiles, c->stream));
+  }
+  { int t = c->q0; c->q0 = c->q1; c->q1 = c->q2; c->q2 = t; }  // Deliver (kernel.cu:921)
+  return RTM_OK;
+}
+
+void build_tiles(rtm_ctx* c, std::vector<int4>& t) {
+  const Grid& g = c->g;
+  for (int z0 = 0; z0 < g.n2; z0 += 16)
+    for (int x0 = 0; x0 < g.nx; x0 += 64) t.push_back(make_int4(z0, x0, 0, 0));
+  for (int z0 = g.nz - g.n2; z0 < g.nz; z0 += 16)
+    for (int x0 = 0; x0 < g.nx; x0 += 64) t.push_back(make_int4(z0, x0, 1, 0));
+  for (int z0 = g.n2; z0 < g.nz - g.n2; z0 += 64) {
+    for (int x0 = 0; x0 < g.n2; x0 += 16) t.push_back(make_int4(z0, x0, 2, 0));
+    for (int x0 = g.nx - g.n2; x0 < g.nx; x0 += 16) t.push_back(make_int4(z0, x0, 3, 0));
+  }
+}
+
+int choose_zs(rtm_ctx* c, int nshot) {
+  const Grid& g = c->g;
+  int zs = c->p.zseg > 0 ? c->p.zseg : 32;
+  c->nxt = (g.mnx + 255) / 256;
+  if (c->p.zseg <= 0) {
+    while (zs > 8) {
+      long long nzg = (g.mnz + 4LL * zs - 1) / (4LL * zs);
+      if ((long long)c->nxt * nzg * nshot >= 1024) break;
+      zs /= 2;
+    }
+  }
+  c->zs = zs;
+  c->nblk = c->nxt * (int)((g.mnz + 4 * zs - 1) / (4 * zs));
+  return RTM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int rtm_create(const rtm_params* p, rtm_ctx** out) {
+  if (!p || !out) return RTM_E_ARG;
+  *out = nullptr;
+  rtm_ctx* c = new (std::nothrow) rtm_ctx();
+  if (!c) return RTM_E_NOMEM;
+  c->p = *p;
+  const int mnz = p->nz - 2 * p->n2, mnx = p->nx - 2 * p->n2;
+  int rc = RTM_OK;
+  if (p->n2 < 1 || mnz < 2 || mnx < 2) rc = fail(c, RTM_E_ARG, "grid too small: nz=%d nx=%d n2=%d", p->nz, p->nx, p->n2);
+  else if (p->nfdmax < 1 || p->nfdmax > 12) rc = fail(c, RTM_E_ARG, "nfdmax=%d outside 1..12", p->nfdmax);
+  else if (p->nfdmax > p->n2) rc = fail(c, RTM_E_ARG, "nfdmax=%d exceeds n2=%d", p->nfdmax, p->n2);
+  else if (p->nt < 3) rc = fail(c, RTM_E_ARG, "nt=%d < 3", p->nt);
+  else if (p->max_shots < 1) rc = fail(c, RTM_E_ARG, "max_shots < 1");
+  else if (p->n < 1 || p->ds < 1) rc = fail(c, RTM_E_ARG, "receivers n=%d ds=%d", p->n, p->ds);
+  else if (p->s_z < 0 || p->s_z >= p->nz || p->s_l < 0 || p->s_r >= p->nx || p->s_r != (p->n - 1) * p->ds + p->s_l)
+    rc = fail(c, RTM_E_ARG, "receiver line outside grid");
+  else if (p->strip_format < 0 || p->strip_format > 2) rc = fail(c, RTM_E_ARG, "strip_format");
+  if (rc != RTM_OK) {
+    int code = rc;
+    delete c;
+    return code;
+  }
+  c->nfd = p->nfdmax;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) { delete c; return RTM_E_HIP; }
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return RTM_E_HIP; }
+
+  Grid& g = c->g;
+  g.nz = p->nz; g.nx = p->nx; g.n2 = p->n2; g.mnz = mnz; g.mnx = mnx; g.nfd = p->nfdmax;
+  const int H = (c->nfd + 3) & ~3;
+  int x0 = std::max(4, H);
+  while ((x0 + g.n2) % 4 != 0) ++x0;
+  g.x0 = x0;
+  g.pitch = ((x0 + g.nx + 4) + 63) / 64 * 64;
+  g.fstride = (long long)g.nz * g.pitch;
+  c->r = Rcv{p->s_l, p->s_r, p->s_z, p->ds, p->n};
+
+  const size_t fb = (size_t)p->max_shots * g.fstride;
+  rc = RTM_OK;
+  for (int i = 0; i < 3 && rc == RTM_OK; ++i) rc = dalloc_t(c, &c->F[i], fb);
+  for (int i = 0; i < 3 && rc == RTM_OK; ++i) rc = dalloc_t(c, &c->Q[i], fb);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->SU, fb);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->SQ, fb);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->I1, fb);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->I2, fb);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_v, (size_t)g.fstride);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_r1, (size_t)g.fstride);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_w, (size_t)g.n2 + 1);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_src, (size_t)p->max_shots);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_data, (size_t)p->max_shots * p->nt * p->n);
+  if (rc == RTM_OK && p->record) rc = dalloc_t(c, &c->d_rec, (size_t)p->max_shots * p->nt * p->n);
+  Strips& st = c->st;
+  st.fmt = p->strip_format;
+  st.nslot = p->nt;
+  st.ss = 2LL * g.nfd * (g.mnz + g.mnx);
+  const size_t esz = st.fmt == RTM_STRIP_F32 ? 4 : 2;
+  if (rc == RTM_OK) rc = dalloc(c, &st.base, (size_t)p->max_shots * st.nslot * st.ss * esz);
+  if (rc == RTM_OK && st.fmt != RTM_STRIP_F32) {
+    rc = dalloc_t(c, &st.scale, (size_t)p->max_shots * st.nslot);
+    if (rc == RTM_OK) rc = dalloc_t(c, &st.amax, (size_t)p->max_shots * st.nslot);
+    if (rc == RTM_OK) rc = dalloc_t(c, &st.stage, (size_t)p->max_shots * 2 * st.ss);
+  }
+  const size_t nI = (size_t)g.mnz * g.mnx;
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_up, (size_t)p->max_shots * nI);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_down, (size_t)p->max_shots * nI);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_s1, nI);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_s2, nI);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_imax, (size_t)p->max_shots);
+  std::vector<int4> tiles;
+  build_tiles(c, tiles);
+  c->ntiles = (int)tiles.size();
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_tiles, tiles.size());
+  if (rc != RTM_OK) {
+    int code = rc;
+    rtm_destroy(c);
+    return code;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_tiles, tiles.data(), tiles.size() * sizeof(int4),
+                           hipMemcpyHostToDevice, c->stream));
+  // Zero everything once so guard columns hold finite values.
+  for (int i = 0; i < 3; ++i) {
+    HIPCHK(c, launch_fill(c->F[i], (long long)fb, 0.f, c->stream));
+    HIPCHK(c, launch_fill(c->Q[i], (long long)fb, 0.f, c->stream));
+  }
+  HIPCHK(c, launch_fill(c->SU, (long long)fb, 0.f, c->stream));
+  HIPCHK(c, launch_fill(c->SQ, (long long)fb, 0.f, c->stream));
+  HIPCHK(c, launch_fill(c->I1, (long long)fb, 0.f, c->stream));
+  HIPCHK(c, launch_fill(c->I2, (long long)fb, 0.f, c->stream));
+  HIPCHK(c, launch_fill(c->d_s1, (long long)nI, 0.f, c->stream));
+  HIPCHK(c, launch_fill(c->d_s2, (long long)nI, 0.f, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = c;
+  return RTM_OK;
+}
+
+int rtm_destroy(rtm_ctx* c) {
+  if (!c) return RTM_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& ep : c->pending) { (void)hipEventDestroy(ep.a); (void)hipEventDestroy(ep.b); }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return RTM_OK;
+}
+
+const char* rtm_last_error(const rtm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+size_t rtm_device_bytes(const rtm_ctx* c) { return c ? c->bytes : 0; }
+
+int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, const float* coef,
+                  int nc, const int* index, int nvel, float vmin, float dv) {
+  if (!c || !v || !r1 || !w || !coef) return RTM_E_ARG;
+  const Grid& g = c->g;
+  const int nfd = c->nfd;
+  const int CS = ((2 + nfd) + 3) & ~3;
+  const bool ls = (c->p.ilste == 0);
+  if (ls && (!index || nvel < 1)) return fail(c, RTM_E_ARG, "LS model needs index[nvel+1]");
+  if (!ls && nc < c->p.nfdmax + 1) return fail(c, RTM_E_ARG, "Taylor table needs nfdmax+1 coefficients");
+  if (!(dv > 0.f)) return fail(c, RTM_E_ARG, "dv must be > 0");
+  const int nrow = ls ? nvel : 1;
+  std::vector<float> tab((size_t)nrow * CS, 0.f);
+  const double h1 = 1.0 + (double)c->p.hzx2_1;
+  for (int k = 0; k < nrow; ++k) {
+    int top = ls ? index[k] : 0;
+    int M = ls ? index[k + 1] - index[k] - 1 : c->p.nfdmax;
+    if (M > nfd) return fail(c, RTM_E_ARG, "bin %d has M=%d > nfdmax=%d", k, M, nfd);
+    if (ls && (top < 0 || index[k + 1] > nc)) return fail(c, RTM_E_ARG, "index[%d] outside c[%d]", k, nc);
+    double c0d = 0.0;
+    if (M >= 0) c0d = h1 * (double)coef[top];  // (1.0+hzx2_1)*Dc[M_top] (kernel.cu:58)
+    memcpy(&tab[(size_t)k * CS], &c0d, sizeof(double));
+    for (int l = 1; l <= M; ++l) tab[(size_t)k * CS + 1 + l] = coef[top + l];
+  }
+  int rc = RTM_OK;
+  if (!c->d_ctab || c->m.nrow != nrow) {
+    rc = dalloc_t(c, &c->d_ctab, tab.size());
+    if (rc != RTM_OK) return rc;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_ctab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  // model arrays into the padded layout; guard columns repeat the edge value
+  std::vector<float> pv((size_t)g.fstride, 0.f), pr((size_t)g.fstride, 0.f);
+  for (int z = 0; z < g.nz; ++z) {
+    float* dv_ = &pv[(size_t)z * g.pitch];
+    float* dr_ = &pr[(size_t)z * g.pitch];
+    for (int x = 0; x < g.pitch; ++x) {
+      int xs = std::min(std::max(x - g.x0, 0), g.nx - 1);
+      dv_[x] = v[(size_t)z * g.nx + xs];
+      dr_[x] = r1[(size_t)z * g.nx + xs];
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_v, pv.data(), pv.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_r1, pr.data(), pr.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_w, w, (g.n2 + 1) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  Model& m = c->m;
+  m.v = c->d_v; m.r1 = c->d_r1; m.ctab = c->d_ctab; m.w = c->d_w;
+  m.cs = CS; m.nrow = nrow; m.ls = ls ? 1 : 0;
+  m.vmin = vmin; m.dv = dv;
+  int ex;
+  float fr = frexpf(dv, &ex);
+  m.dv_pow2 = (fr == 0.5f) ? 1 : 0;
+  m.inv_dv = m.dv_pow2 ? ldexpf(1.0f, 1 - ex) : 1.0f / dv;
+  m.hzx2_1 = c->p.hzx2_1; m.tao2 = c->p.tao2; m.h2 = c->p.h2;
+  m.taoh = c->p.taoh; m.taoh2 = c->p.taoh2;
+  c->model_ok = true;
+  return RTM_OK;
+}
+
+int rtm_set_wavelet(rtm_ctx* c, const float* wav) {
+  if (!c || !wav) return RTM_E_ARG;
+  c->wav.assign(wav, wav + c->p.nt + 1);
+  c->wav_ok = true;
+  return RTM_OK;
+}
+
+int rtm_set_shots(rtm_ctx* c, int nshot, const int* r_u, const int* r_x) {
+  if (!c || !r_u || !r_x) return RTM_E_ARG;
+  if (nshot < 1 || nshot > c->p.max_shots) return fail(c, RTM_E_ARG, "nshot=%d outside 1..%d", nshot, c->p.max_shots);
+  c->src.resize(nshot);
+  for (int i = 0; i < nshot; ++i) {
+    if (r_u[i] < 0 || r_u[i] >= c->g.nz || r_x[i] < 0 || r_x[i] >= c->g.nx)
+      return fail(c, RTM_E_ARG, "shot %d source (%d,%d) outside grid", i, r_u[i], r_x[i]);
+    c->src[i] = make_int2(r_u[i], r_x[i]);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_src, c->src.data(), nshot * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->nshot = nshot;
+  choose_zs(c, nshot);
+  c->shots_ok = true;
+  c->data_ok = false;
+  c->kf = -1;
+  c->kb = -2;
+  return RTM_OK;
+}
+
+int rtm_set_gathers(rtm_ctx* c, const float* seis) {
+  if (!c || !seis) return RTM_E_ARG;
+  if (!c->shots_ok) return fail(c, RTM_E_STATE, "rtm_set_shots first");
+  const int n = c->p.n, nt = c->p.nt;
+  std::vector<float> t((size_t)nt * n);
+  for (int s = 0; s < c->nshot; ++s) {
+    const float* src = seis + (size_t)s * n * nt;
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < nt; ++k) t[(size_t)k * n + i] = src[(size_t)i * nt + k];
+    HIPCHK(c, hipMemcpy(c->d_data + (size_t)s * nt * n, t.data(), t.size() * sizeof(float),
+                        hipMemcpyHostToDevice));
+  }
+  c->data_ok = true;
+  return RTM_OK;
+}
+
+int rtm_forward_begin(rtm_ctx* c, int save_strips) {
+  if (!c) return RTM_E_ARG;
+  if (!c->model_ok || !c->wav_ok || !c->shots_ok) return fail(c, RTM_E_STATE, "model, wavelet and shots must be set");
+  c->f0 = 0; c->f1 = 1; c->f2 = 2;
+  c->save_strips = save_strips != 0;
+  const float spike = (float)((double)c->wav[0] / 2.0);  // f(0.0,f0)/2.0 (kernel.cu:803)
+  HIPCHK(c, launch_fwd_begin(c->g, c->F[0], c->F[1], c->d_src, spike, c->nshot, c->stream));
+  if (c->save_strips) {  // Equal (kernel.cu:808)
+    if (c->st.fmt != RTM_STRIP_F32)
+      HIPCHK(c, launch_fill(reinterpret_cast<float*>(c->st.amax), (long long)c->nshot * c->st.nslot, 0.f, c->stream));
+    HIPCHK(c, launch_strip_save(c->g, c->F[0], c->st, 0, c->nshot, c->stream));
+    HIPCHK(c, launch_strip_save(c->g, c->F[1], c->st, 1, c->nshot, c->stream));
+    if (c->st.fmt != RTM_STRIP_F32) {
+      HIPCHK(c, launch_strip_finish(c->g, c->st, 0, c->nshot, c->stream));
+      HIPCHK(c, launch_strip_finish(c->g, c->st, 1, c->nshot, c->stream));
+    }
+  }
+  if (c->p.record) {
+    HIPCHK(c, launch_record(c->g, c->r, c->F[0], c->d_rec, 0, c->p.nt, c->nshot, c->stream));
+    HIPCHK(c, launch_record(c->g, c->r, c->F[1], c->d_rec, 1, c->p.nt, c->nshot, c->stream));
+  }
+  c->kf = 2;
+  c->kb = -2;
+  return RTM_OK;
+}
+
+int rtm_forward_advance(rtm_ctx* c, int nsteps) {
+  if (!c) return RTM_E_ARG;
+  if (c->kf < 2) return fail(c, RTM_E_STATE, "rtm_forward_begin first");
+  for (int i = 0; i < nsteps && c->kf < c->p.nt; ++i, ++c->kf) {
+    int rc = forward_step(c, c->kf);
+    if (rc != RTM_OK) return rc;
+  }
+  return RTM_OK;
+}
+
+int rtm_backward_begin(rtm_ctx* c) {
+  if (!c) return RTM_E_ARG;
+  if (c->kf != c->p.nt) return fail(c, RTM_E_STATE, "forward pass incomplete (k=%d of %d)", c->kf, c->p.nt);
+  if (!c->save_strips) return fail(c, RTM_E_STATE, "forward pass ran without strips");
+  if (!c->data_ok) return fail(c, RTM_E_STATE, "rtm_set_gathers first");
+  // DBW1 = u(NT-2), DBW0 = u(NT-1) (kernel.cu:822-825): swap roles of f0/f1.
+  std::swap(c->f0, c->f1);
+  c->q0 = 0; c->q1 = 1; c->q2 = 2;
+  const float spike = (float)((double)c->wav[0] / 2.0);
+  HIPCHK(c, launch_bwd_begin(c->g, c->F[c->f0], c->F[c->f1], c->Q[0], c->Q[1], c->SU, c->SQ,
+                             c->I1, c->I2, c->d_src, spike, c->p.icompen, c->nshot, c->stream));
+  // BKEqual of the first backward step (kernel.cu:892, k = NT-3): slot NT-2.
+  HIPCHK(c, launch_strip_restore(c->g, c->F[c->f1], c->st, c->p.nt - 2, c->nshot, c->stream));
+  c->kb = c->p.nt - 3;
+  return RTM_OK;
+}
+
+int rtm_backward_advance(rtm_ctx* c, int nsteps) {
+  if (!c) return RTM_E_ARG;
+  if (c->kb < -1) return fail(c, RTM_E_STATE, "rtm_backward_begin first");
+  for (int i = 0; i < nsteps && c->kb >= 0; ++i, --c->kb) {
+    int rc = backward_step(c, c->kb);
+    if (rc != RTM_OK) return rc;
+  }
+  return RTM_OK;
+}
+
+int rtm_migrate(rtm_ctx* c) {
+  int rc = rtm_forward_begin(c, 1);
+  if (rc == RTM_OK) rc = rtm_forward_advance(c, c->p.nt);
+  if (rc == RTM_OK) rc = rtm_backward_begin(c);
+  if (rc == RTM_OK) rc = rtm_backward_advance(c, c->p.nt);
+  return rc;
+}
+
+int rtm_sync(rtm_ctx* c) {
+  if (!c) return RTM_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RTM_OK;
+}
+
+int rtm_read_field(rtm_ctx* c, int shot, int which, float* out) {
+  if (!c || !out) return RTM_E_ARG;
+  if (shot < 0 || shot >= c->nshot) return fail(c, RTM_E_ARG, "shot %d", shot);
+  const float* base = nullptr;
+  switch (which) {
+    case RTM_FIELD_U: base = c->F[c->f1]; break;
+    case RTM_FIELD_U_PREV: base = c->F[c->f0]; break;
+    case RTM_FIELD_Q: base = c->Q[c->q1]; break;
+    case RTM_FIELD_I1: base = c->I1; break;
+    case RTM_FIELD_I2: base = c->I2; break;
+    case RTM_FIELD_SU: base = c->SU; break;
+    case RTM_FIELD_SQ: base = c->SQ; break;
+    default: return fail(c, RTM_E_ARG, "field %d", which);
+  }
+  const Grid& g = c->g;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy2D(out, g.nx * sizeof(float), base + (size_t)shot * g.fstride + g.x0,
+                        g.pitch * sizeof(float), g.nx * sizeof(float), g.nz, hipMemcpyDeviceToHost));
+  if (which >= RTM_FIELD_I1) {  // images are defined on the interior only
+    for (int z = 0; z < g.nz; ++z)
+      for (int x = 0; x < g.nx; ++x)
+        if (z < g.n2 || z >= g.nz - g.n2 || x < g.n2 || x >= g.nx - g.n2) out[(size_t)z * g.nx + x] = 0.f;
+  }
+  return RTM_OK;
+}
+
+int rtm_read_records(rtm_ctx* c, float* out) {
+  if (!c || !out) return RTM_E_ARG;
+  if (!c->p.record) return fail(c, RTM_E_STATE, "context created without records");
+  const int n = c->p.n, nt = c->p.nt;
+  std::vector<float> t((size_t)nt * n);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < c->nshot; ++s) {
+    HIPCHK(c, hipMemcpy(t.data(), c->d_rec + (size_t)s * nt * n, t.size() * sizeof(float), hipMemcpyDeviceToHost));
+    float* o = out + (size_t)s * n * nt;
+    for (int k = 0; k < nt; ++k)
+      for (int i = 0; i < n; ++i) o[(size_t)i * nt + k] = t[(size_t)k * n + i];
+  }
+  return RTM_OK;
+}
+
+int rtm_read_strip(rtm_ctx* c, int shot, int k, float* out) {
+  if (!c || !out) return RTM_E_ARG;
+  if (shot < 0 || shot >= c->nshot || k < 0 || k >= c->p.nt) return fail(c, RTM_E_ARG, "shot/k");
+  const Strips& st = c->st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const long long slot = (long long)shot * st.nslot + (k % st.nslot);
+  if (st.fmt == RTM_STRIP_F32) {
+    HIPCHK(c, hipMemcpy(out, (const float*)st.base + slot * st.ss, st.ss * sizeof(float), hipMemcpyDeviceToHost));
+    return RTM_OK;
+  }
+  std::vector<unsigned short> h(st.ss);
+  float sc = 1.f;
+  HIPCHK(c, hipMemcpy(h.data(), (const unsigned short*)st.base + slot * st.ss, st.ss * 2, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&sc, st.scale + slot, sizeof(float), hipMemcpyDeviceToHost));
+  for (long long e = 0; e < st.ss; ++e) {
+    unsigned short b = h[e];
+    float f;
+    if (st.fmt == RTM_STRIP_BF16) {
+      unsigned u = ((unsigned)b) << 16;
+      memcpy(&f, &u, 4);
+    } else {
+      f = half_to_float(b);
+    }
+    out[e] = f * sc;
+  }
+  return RTM_OK;
+}
+
+int rtm_image_post(rtm_ctx* c, float vmax, float whitecoe, float* up, float* down) {
+  if (!c) return RTM_E_ARG;
+  if (c->kb != -1) return fail(c, RTM_E_STATE, "backward pass incomplete");
+  const Grid& g = c->g;
+  const int nI = g.mnz * g.mnx;
+  HIPCHK(c, launch_fill(reinterpret_cast<float*>(c->d_imax), c->nshot, 0.f, c->stream));
+  HIPCHK(c, launch_maxabs(g, c->I2, c->d_imax, c->nshot, c->stream));
+  HIPCHK(c, launch_post(g, c->d_v, c->I1, c->I2, c->d_imax, vmax, whitecoe, c->d_up, c->d_down,
+                        c->nshot, c->stream));
+  HIPCHK(c, launch_stack_add(nI, c->d_up, c->d_down, c->d_s1, c->d_s2, c->nshot, c->stream));
+  if (up) HIPCHK(c, hipMemcpyAsync(up, c->d_up, sizeof(float) * nI * c->nshot, hipMemcpyDeviceToHost, c->stream));
+  if (down) HIPCHK(c, hipMemcpyAsync(down, c->d_down, sizeof(float) * nI * c->nshot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RTM_OK;
+}
+
+int rtm_stack_finish(rtm_ctx* c, int nrec, int inorm, float* img, float* illum) {
+  if (!c || !img || nrec < 1) return RTM_E_ARG;
+  const int nI = c->g.mnz * c->g.mnx;
+  // d_up/d_down of shot 0 are reused as the result buffers.
+  HIPCHK(c, launch_stack_finish(nI, c->d_s1, c->d_s2, nrec, inorm, c->d_up, c->d_down, c->stream));
+  HIPCHK(c, hipMemcpyAsync(img, c->d_up, sizeof(float) * nI, hipMemcpyDeviceToHost, c->stream));
+  if (illum) HIPCHK(c, hipMemcpyAsync(illum, c->d_down, sizeof(float) * nI, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, launch_fill(c->d_s1, nI, 0.f, c->stream));
+  HIPCHK(c, launch_fill(c->d_s2, nI, 0.f, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RTM_OK;
+}
+
+int rtm_set_timing(rtm_ctx* c, int on) {
+  if (!c) return RTM_E_ARG;
+  int rc = drain_timing(c);
+  if (rc != RTM_OK) return rc;
+  c->timing = on != 0;
+  for (int i = 0; i < RTM_K_NKINDS; ++i) { c->t_ms[i] = 0; c->t_n[i] = 0; }
+  return RTM_OK;
+}
+
+int rtm_kernel_time(rtm_ctx* c, int kind, double* total_ms, long long* launches) {
+  if (!c || kind < 0 || kind >= RTM_K_NKINDS) return RTM_E_ARG;
+  int rc = drain_timing(c);
+  if (rc != RTM_OK) return rc;
+  if (total_ms) *total_ms = c->t_ms[kind];
+  if (launches) *launches = c->t_n[kind];
+  return RTM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// rtm_kernels.h -- device-side data structures shared by the gfx950 kernels
+// (rtm_kernels.hip) and the C-ABI glue (rtm_capi.hip).  Internal header.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rtm {
+
+// Padded-grid geometry in device memory.  Cell (z, x) of shot s in a field
+// array lives at s*fstride + z*pitch + x0 + x.  x0 is chosen so that the
+// first interior column n2 is 16-byte aligned, and the row has >= 4 guard
+// columns on each side so every float4 row access stays inside its row.
+struct Grid {
+  int nz, nx, pitch, x0, n2, mnz, mnx;
+  int nfd;          // nfdmax of the model (strip depth)
+  long long fstride;
+};
+
+// Model operands.  ctab rows are [c0' as double | c_1 .. c_NFD | zero pad],
+// c0' = (1.0 + hzx2_1) * c_0 in double (the promotion of kernel.cu:58).
+struct Model {
+  const float* v;
+  const float* r1;
+  const float* ctab;
+  const float* w;
+  int cs, nrow, ls, dv_pow2;
+  float vmin, dv, inv_dv;
+  float hzx2_1, tao2, h2, taoh, taoh2;
+};
+
+struct Rcv {
+  int s_l, s_r, s_z, ds, n;
+};
+
+// Strip storage: fp32, or fp16/bf16 with a per-(shot, slot) scale.
+struct Strips {
+  void* base;          // [shot][slot][ss] elements
+  float* scale;        // [shot][slot] (fp16/bf16)
+  unsigned* amax;      // [shot][slot] max |value| bits (fp16/bf16)
+  float* stage;        // [shot][2][ss] fp32 staging (fp16/bf16)
+  int fmt, nslot;
+  long long ss;
+};
+
+enum { EPI_FWD = 0, EPI_SRC = 1, EPI_RCV = 2 };
+enum { FR_FWD = 0, FR_RCV = 1 };
+
+struct IntArgs {
+  Grid g;
+  Model m;
+  Rcv r;
+  const float* u1;
+  const float* u0;
+  float* u2;
+  const int2* src;     // per shot (r_u, r_x)
+  float wav;           // source amplitude this step (0 past the gate)
+  int k, nt;
+  int nxt, zs;         // interior x tiles, rows per wave
+  int f64comb;         // forward: 1 when ilste = 1 (Add_Con)
+  float* rec;          // [shot][nt][n] or null
+  const float* data;   // [shot][nt][n] (receiver mode)
+  const float* uk;     // u(k) for imaging (receiver mode)
+  float *su, *sq, *i1, *i2;
+  int icompen;
+};
+
+struct FrameArgs {
+  Grid g;
+  Model m;
+  Rcv r;
+  const float* u1;
+  const float* u0;
+  float* u2;
+  const int4* tiles;   // (z0, x0, band, 0)
+  const int2* src;
+  float wav;
+  int k, nt;
+  int f64comb;
+  float* rec;
+  const float* data;
+  Strips st;
+  int save_strips;
+};
+
+// Launchers (rtm_kernels.hip).  nfd_inst is the compile-time stencil
+// half-length the call is dispatched to (>= model nfdmax, <= n2).
+hipError_t launch_interior(int epi, int nfd_inst, const IntArgs& a, int nshot, int nblk,
+                           hipStream_t s);
+hipError_t launch_frame(int mode, int nfd_inst, const FrameArgs& a, int nshot, int ntiles,
+                        hipStream_t s);
+hipError_t launch_fill(float* p, long long n, float val, hipStream_t s);
+hipError_t launch_fwd_begin(const Grid& g, float* f0, float* f1, const int2* src, float spike,
+                            int nshot, hipStream_t s);
+hipError_t launch_strip_save(const Grid& g, const float* f, Strips st, int k, int nshot,
+                             hipStream_t s);
+hipError_t launch_strip_restore(const Grid& g, float* f, Strips st, int k, int nshot,
+                                hipStream_t s);
+hipError_t launch_strip_finish(const Grid& g, Strips st, int k, int nshot, hipStream_t s);
+hipError_t launch_record(const Grid& g, const Rcv& r, const float* f, float* rec, int k, int nt,
+                         int nshot, hipStream_t s);
+hipError_t launch_bwd_begin(const Grid& g, const float* s0, const float* s1, float* q0,
+                            float* q1, float* su, float* sq, float* i1, float* i2,
+                            const int2* src, float spike, int icompen, int nshot, hipStream_t s);
+hipError_t launch_maxabs(const Grid& g, const float* i2, unsigned* out, int nshot, hipStream_t s);
+hipError_t launch_post(const Grid& g, const float* v, const float* i1, const float* i2,
+                       const unsigned* amax, float vmax, float whitecoe, float* up, float* down,
+                       int nshot, hipStream_t s);
+hipError_t launch_stack_add(int nI, const float* up, const float* down, float* s1, float* s2,
+                            int nshot, hipStream_t s);
+hipError_t launch_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
+                               float* img, float* illum, hipStream_t s);
+
+}  // namespace rtm

@@ -1,0 +1,752 @@
+// rtm_kernels.hip -- hand-written gfx950 kernels of the 2-D acoustic RTM step.
+//
+// Reference kernels (caixh90/RTM_GPU kernel.cu) and where their work lives here:
+//   Add/Add_Con (46-114)            -> k_interior<EPI_FWD> + k_frame<FR_FWD> (two-way part)
+//   Hybrid1/Hybrid2 (116-183)       -> k_frame<FR_FWD> (one-way + blend, from pre-blend values)
+//   Equal/Hybrid3 (18-45, 184-208)  -> k_strip_save, and k_frame<FR_FWD> for k >= 2
+//   Deliver/Deliver_EFF (210-221, 323-337) -> none: the host rotates buffer pointers
+//   BKEqual (222-245)               -> k_strip_restore (into the buffer that becomes u(k+1))
+//   BKAdd_EFF(_Con) (246-320)       -> k_interior<EPI_SRC>
+//   BKAdd(_Con) (339-418)           -> k_interior<EPI_RCV> + k_frame<FR_RCV>
+//   BKHybrid1/2 (421-487)           -> k_frame<FR_RCV>
+//   Rel_Compen/Rel_NonCompen (489-517) -> k_interior<EPI_RCV> epilogue (interior only)
+//
+// Arithmetic follows the reference formulas operation by operation (build
+// flags: -ffp-contract=off, IEEE division, denormals kept), with explicit
+// fused multiply-adds only at the per-tap stencil sum, the tap accumulation,
+// the FP32 final combine and the imaging accumulations.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "rtm_kernels.h"
+
+namespace rtm {
+
+// ---------------------------------------------------------------------------
+// helpers
+template <int I>
+__device__ __forceinline__ float comp(const float4& v) {
+  static_assert(I >= 0 && I < 4, "component");
+  return I == 0 ? v.x : (I == 1 ? v.y : (I == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ float compr(const float4& v, int i) {
+  return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Velocity bin of kernel.cu:55-56: FP32 quotient, double + 0.5, truncate.
+// Clamped to the table so that guard cells can never index outside it.
+__device__ __forceinline__ int bin_of(float v, const Model& m) {
+  if (!m.ls) return 0;
+  float q = m.dv_pow2 ? (v - m.vmin) * m.inv_dv : (v - m.vmin) / m.dv;
+  int k = (int)((double)q + 0.5);
+  k = k < 0 ? 0 : k;
+  k = k >= m.nrow ? m.nrow - 1 : k;
+  return k;
+}
+
+template <int NFD>
+struct CoefRow {
+  static constexpr int CS = ((2 + NFD) + 3) & ~3;
+  static constexpr int NR = CS / 4;
+  float4 r[NR];
+  __device__ __forceinline__ void load(const float* ctab, int kb) {
+    const float4* p = reinterpret_cast<const float4*>(ctab + (long)kb * CS);
+#pragma unroll
+    for (int i = 0; i < NR; ++i) r[i] = p[i];
+  }
+  __device__ __forceinline__ double c0d() const {
+    return __hiloint2double(__float_as_int(r[0].y), __float_as_int(r[0].x));
+  }
+  // tap coefficient c_l, l >= 1 (float index 1 + l in the row)
+  __device__ __forceinline__ float c(int l) const { return compr(r[(1 + l) >> 2], (1 + l) & 3); }
+};
+
+// Final combine of the two-way update.
+//  FP32 form (Add, BKAdd, BKAdd_Con: kernel.cu:73,375,413): 2*u1-u0+P.
+//  FP64 form (Add_Con, BKAdd_EFF, BKAdd_EFF_Con: kernel.cu:106,275,312): 2.0*u1-u0+P.
+__device__ __forceinline__ float combine(float u, float u0, float vv, float w1, const Model& m,
+                                         bool f64) {
+  float p = vv * vv * m.tao2 * m.h2;
+  if (f64) return (float)((2.0 * (double)u - (double)u0) + (double)(p * w1));
+  return __builtin_fmaf(p, w1, 2.0f * u - u0);
+}
+
+__device__ __forceinline__ bool is_receiver(int c, const Rcv& r, int* s) {
+  if (c < r.s_l || c > r.s_r) return false;
+  int d = c - r.s_l;
+  if (d % r.ds != 0) return false;
+  *s = d / r.ds;
+  return true;
+}
+
+__device__ __forceinline__ int mirror_idx(int g, int n) {  // kernel.cu:65-68
+  if (g < 0) g = -g;
+  if (g >= n) g = 2 * n - 2 - g;
+  g = g < 0 ? 0 : g;
+  g = g >= n ? n - 1 : g;
+  return g;
+}
+
+// ---- strip encoding -------------------------------------------------------
+__device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest even
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float(((unsigned)b) << 16); }
+
+// Strip element index -> grid cell.  Slot layout: lf[mnz][nfd], rt[mnz][nfd],
+// up[nfd][mnx], dw[nfd][mnx] (Dlf/Drt/Dup/Ddw of kernel.cu:199-205).
+__device__ __forceinline__ void strip_cell(const Grid& g, long long e, int* z, int* x) {
+  const long long a = (long long)g.mnz * g.nfd, b = (long long)g.nfd * g.mnx;
+  if (e < a) {
+    int r = (int)(e / g.nfd), d = (int)(e % g.nfd);
+    *z = g.n2 + r; *x = g.n2 - 1 - d;
+  } else if (e < 2 * a) {
+    e -= a;
+    int r = (int)(e / g.nfd), d = (int)(e % g.nfd);
+    *z = g.n2 + r; *x = g.nx - g.n2 + d;
+  } else if (e < 2 * a + b) {
+    e -= 2 * a;
+    int d = (int)(e / g.mnx), c = (int)(e % g.mnx);
+    *z = g.n2 - 1 - d; *x = g.n2 + c;
+  } else {
+    e -= 2 * a + b;
+    int d = (int)(e / g.mnx), c = (int)(e % g.mnx);
+    *z = g.nz - g.n2 + d; *x = g.n2 + c;
+  }
+}
+// Grid cell -> strip element index, or -1.
+__device__ __forceinline__ long long strip_index(const Grid& g, int z, int x) {
+  const long long a = (long long)g.mnz * g.nfd, b = (long long)g.nfd * g.mnx;
+  const int n2 = g.n2, nfd = g.nfd;
+  if (z >= n2 && z < g.nz - n2) {
+    if (x >= n2 - nfd && x < n2) return (long long)(z - n2) * nfd + (n2 - 1 - x);
+    if (x >= g.nx - n2 && x < g.nx - n2 + nfd) return a + (long long)(z - n2) * nfd + (x - (g.nx - n2));
+    return -1;
+  }
+  if (x >= n2 && x < g.nx - n2) {
+    if (z >= n2 - nfd && z < n2) return 2 * a + (long long)(n2 - 1 - z) * g.mnx + (x - n2);
+    if (z >= g.nz - n2 && z < g.nz - n2 + nfd) return 2 * a + b + (long long)(z - (g.nz - n2)) * g.mnx + (x - n2);
+  }
+  return -1;
+}
+__device__ __forceinline__ void strip_put(const Strips& st, int shot, int k, long long e, float val) {
+  if (st.fmt == 0) {
+    float* p = (float*)st.base;
+    p[((long long)shot * st.nslot + (k % st.nslot)) * st.ss + e] = val;
+  } else {
+    st.stage[((long long)shot * 2 + (k & 1)) * st.ss + e] = val;
+    atomicMax(&st.amax[(long long)shot * st.nslot + (k % st.nslot)], __float_as_uint(fabsf(val)));
+  }
+}
+__device__ __forceinline__ float strip_get(const Strips& st, int shot, int k, long long e) {
+  long long slot = (long long)shot * st.nslot + (k % st.nslot);
+  if (st.fmt == 0) return ((const float*)st.base)[slot * st.ss + e];
+  const unsigned short* p = (const unsigned short*)st.base;
+  unsigned short h = p[slot * st.ss + e];
+  float sc = st.scale[slot];
+  if (st.fmt == 1) return __half2float(__ushort_as_half(h)) * sc;
+  return bf2f(h) * sc;
+}
+
+// ---------------------------------------------------------------------------
+// Interior update: cells [n2, nz-n2) x [n2, nx-n2).  One wave owns 256
+// columns (4 per lane, float4 rows) and marches down a segment of rows.
+// Vertical taps come from a register queue of rows (row z+NFD+1 is fetched
+// one row ahead); horizontal taps from a per-wave LDS row with a halo of H
+// columns each side.  No mirror is needed: every tap stays inside the grid
+// because NFD <= n2.
+template <int NFD, int EPI>
+__global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
+  constexpr int H = (NFD + 3) & ~3;
+  constexpr int QL = 2 * NFD + 2;
+  constexpr int W4 = 64 + 2 * (H / 4);
+  constexpr int NWIN = 1 + 2 * (H / 4);
+  __shared__ float4 lds[4][W4];
+
+  const Grid& g = a.g;
+  const Model& m = a.m;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int xt = blockIdx.x % a.nxt, zg = blockIdx.x / a.nxt;
+  const int shot = blockIdx.y;
+  const int zhi = g.nz - g.n2;
+  const int zs = g.n2 + (zg * 4 + wv) * a.zs;
+  if (zs >= zhi) return;
+  const int ze = min(zs + a.zs, zhi);
+  const int cend = g.nx - g.n2;                 // first column past the interior
+  const int cb = g.n2 + xt * 256;
+  const int c0 = cb + lane * 4;
+  const int clast = g.nx - ((g.nx - g.n2) & 3); // largest c == n2 (mod 4) with c <= nx
+  const int cl = min(c0, clast);
+  const long long so = (long long)shot * g.fstride;
+  const float* __restrict__ U1 = a.u1 + so + g.x0;
+  const float* __restrict__ U0 = a.u0 + so + g.x0;
+  float* __restrict__ U2 = a.u2 + so + g.x0;
+  const float* __restrict__ V = m.v + g.x0;
+  const int P = g.pitch;
+  auto ld = [&](const float* base, int z, int c) -> float4 {
+    return *reinterpret_cast<const float4*>(base + (long)z * P + c);
+  };
+
+  const bool hlane = lane < 2 * (H / 4);
+  int hcol = 0, hpos = 0;
+  if (hlane) {
+    if (lane < H / 4) { hcol = cb - H + 4 * lane; hpos = lane; }
+    else { hcol = cb + 256 + 4 * (lane - H / 4); hpos = 64 + lane; }
+    hcol = min(hcol, clast);
+  }
+  const int2 sp = a.src ? a.src[shot] : make_int2(-1, -1);
+
+  float4 q[QL];
+#pragma unroll
+  for (int s = 0; s < 2 * NFD + 1; ++s) q[s] = ld(U1, zs - NFD + s, cl);
+
+  for (int z = zs; z < ze; z += QL) {
+#pragma unroll
+    for (int j = 0; j < QL; ++j) {
+      const int zz = z + j;
+      if (zz < ze) {
+        q[(j + QL - 1) % QL] = ld(U1, min(zz + NFD + 1, g.nz - 1), cl);
+        float4 hv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (hlane) hv = ld(U1, zz, hcol);
+        const float4 u0v = ld(U0, zz, cl);
+        const float4 vv = ld(V, zz, cl);
+        const float4 qc = q[(j + NFD) % QL];
+        lds[wv][H / 4 + lane] = qc;
+        if (hlane) lds[wv][hpos] = hv;
+        wave_lds_sync();
+        float4 win[NWIN];
+#pragma unroll
+        for (int t = 0; t < NWIN; ++t) win[t] = lds[wv][lane + t];
+        wave_lds_sync();
+
+        float out[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const float vj = compr(vv, jj);
+          CoefRow<NFD> cr;
+          cr.load(m.ctab, bin_of(vj, m));
+          const float u = compr(qc, jj);
+          float w1 = (float)(cr.c0d() * (double)u);
+#pragma unroll
+          for (int l = 1; l <= NFD; ++l) {
+            float s = compr(q[(j + NFD - l) % QL], jj) + compr(q[(j + NFD + l) % QL], jj);
+            s = __builtin_fmaf(s, m.hzx2_1, compr(win[(H + jj - l) >> 2], (H + jj - l) & 3));
+            s = s + compr(win[(H + jj + l) >> 2], (H + jj + l) & 3);
+            w1 = __builtin_fmaf(cr.c(l), s, w1);
+          }
+          const bool f64 = (EPI == EPI_SRC) || (EPI == EPI_FWD && a.f64comb);
+          out[jj] = combine(u, compr(u0v, jj), vj, w1, m, f64);
+        }
+
+        if (EPI == EPI_FWD || EPI == EPI_SRC) {
+          if (zz == sp.x) {
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+              if (c0 + jj == sp.y) out[jj] = out[jj] + a.wav;  // kernel.cu:74-77, 277-280
+          }
+        }
+        if (EPI == EPI_RCV && zz == a.r.s_z) {  // data replacement (kernel.cu:349-354)
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            int s;
+            if (c0 + jj < cend && is_receiver(c0 + jj, a.r, &s)) {
+              float d = a.data[((long long)shot * a.nt + (a.k + 1)) * a.r.n + s];
+              if (d != 0.0f) out[jj] = d;
+            }
+          }
+        }
+        float* dst = U2 + (long)zz * P + c0;
+        if (c0 + 3 < cend) {
+          *reinterpret_cast<float4*>(dst) = make_float4(out[0], out[1], out[2], out[3]);
+        } else {
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj)
+            if (c0 + jj < cend) dst[jj] = out[jj];
+        }
+        if (EPI == EPI_FWD && a.rec && zz == a.r.s_z) {
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj) {
+            int s;
+            if (c0 + jj < cend && is_receiver(c0 + jj, a.r, &s))
+              a.rec[((long long)shot * a.nt + a.k) * a.r.n + s] = out[jj];
+          }
+        }
+        if (EPI == EPI_RCV) {  // imaging (kernel.cu:489-517), interior only
+          const long long off = so + g.x0 + (long)zz * P + cl;
+          const float4 ukv = *reinterpret_cast<const float4*>(a.uk + off);
+          float4 i1 = *reinterpret_cast<const float4*>(a.i1 + off);
+          float4 i2 = *reinterpret_cast<const float4*>(a.i2 + off);
+          float r1o[4], r2o[4], rsu[4], rsq[4];
+          if (a.icompen) {
+            float4 su = *reinterpret_cast<const float4*>(a.su + off);
+            float4 sq = *reinterpret_cast<const float4*>(a.sq + off);
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+              const float u = compr(ukv, jj);
+              rsu[jj] = compr(su, jj) + u;
+              rsq[jj] = compr(sq, jj) + out[jj];
+              r1o[jj] = __builtin_fmaf(rsq[jj], rsu[jj], compr(i1, jj));
+              r2o[jj] = __builtin_fmaf(u, u, compr(i2, jj));
+            }
+          } else {
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+              const float u = compr(ukv, jj);
+              r1o[jj] = __builtin_fmaf(out[jj], u, compr(i1, jj));
+              r2o[jj] = __builtin_fmaf(u, u, compr(i2, jj));
+              rsu[jj] = rsq[jj] = 0.f;
+            }
+          }
+          const long long o2 = so + g.x0 + (long)zz * P + c0;
+          if (c0 + 3 < cend) {
+            *reinterpret_cast<float4*>(a.i1 + o2) = make_float4(r1o[0], r1o[1], r1o[2], r1o[3]);
+            *reinterpret_cast<float4*>(a.i2 + o2) = make_float4(r2o[0], r2o[1], r2o[2], r2o[3]);
+            if (a.icompen) {
+              *reinterpret_cast<float4*>(a.su + o2) = make_float4(rsu[0], rsu[1], rsu[2], rsu[3]);
+              *reinterpret_cast<float4*>(a.sq + o2) = make_float4(rsq[0], rsq[1], rsq[2], rsq[3]);
+            }
+          } else {
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+              if (c0 + jj < cend) {
+                a.i1[o2 + jj] = r1o[jj];
+                a.i2[o2 + jj] = r2o[jj];
+                if (a.icompen) { a.su[o2 + jj] = rsu[jj]; a.sq[o2 + jj] = rsq[jj]; }
+              }
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Frame update (hybrid ABC).  A tile of TZ x TX cells of one band:
+//   band 0 top    rows [0, n2)         (16 x 64 tiles)
+//   band 1 bottom rows [nz-n2, nz)     (16 x 64)
+//   band 2 left   rows [n2, nz-n2), cols [0, n2)      (64 x 16)
+//   band 3 right  rows [n2, nz-n2), cols [nx-n2, nx)  (64 x 16)
+// Phase 1 stages u1 on the tile plus a (1+NFD) halo in LDS with the outer
+// mirror of kernel.cu:65-68 applied at load time.  Phase 2 computes the
+// pre-blend two-way value T (with source / data) on the tile plus a 1-cell
+// halo.  Phase 3 computes the one-way value of each ring cell from T, u1, u0
+// (Hybrid1, kernel.cu:116-158) and blends (Hybrid2, kernel.cu:160-183).
+template <int NFD, int MODE>
+__global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
+  constexpr int RMAX = (16 + 2 + 2 * NFD) * (64 + 2 + 2 * NFD);
+  constexpr int TMAX = (16 + 2) * (64 + 2);
+  __shared__ float su1[RMAX];
+  __shared__ float sT[TMAX];
+  const Grid& g = a.g;
+  const Model& m = a.m;
+  const int4 td = a.tiles[blockIdx.x];
+  const int shot = blockIdx.y;
+  const int band = td.z;
+  const int TZ = band < 2 ? 16 : 64, TX = band < 2 ? 64 : 16;
+  const int RZ = TZ + 2 + 2 * NFD, RX = TX + 2 + 2 * NFD;
+  const int TRZ = TZ + 2, TRX = TX + 2;
+  const int z0 = td.x, x0 = td.y;
+  const long long so = (long long)shot * g.fstride + g.x0;
+  const float* __restrict__ U1 = a.u1 + so;
+  const float* __restrict__ U0 = a.u0 + so;
+  float* __restrict__ U2 = a.u2 + so;
+  const float* __restrict__ V = m.v + g.x0;
+  const float* __restrict__ R1 = m.r1 + g.x0;
+  const int P = g.pitch;
+  const int2 sp = a.src ? a.src[shot] : make_int2(-1, -1);
+
+  for (int idx = threadIdx.x; idx < RZ * RX; idx += 256) {
+    int i = idx / RX, jx = idx - i * RX;
+    int gz = mirror_idx(z0 - 1 - NFD + i, g.nz);
+    int gx = mirror_idx(x0 - 1 - NFD + jx, g.nx);
+    su1[idx] = U1[(long)gz * P + gx];
+  }
+  __syncthreads();
+
+  for (int idx = threadIdx.x; idx < TRZ * TRX; idx += 256) {
+    int i = idx / TRX, jx = idx - i * TRX;
+    int gz = z0 - 1 + i, gx = x0 - 1 + jx;
+    float T = 0.f;
+    if (gz >= 0 && gz < g.nz && gx >= 0 && gx < g.nx) {
+      const int ci = (i + NFD) * RX + (jx + NFD);
+      const float vj = V[(long)gz * P + gx];
+      CoefRow<NFD> cr;
+      cr.load(m.ctab, bin_of(vj, m));
+      const float u = su1[ci];
+      float w1 = (float)(cr.c0d() * (double)u);
+#pragma unroll
+      for (int l = 1; l <= NFD; ++l) {
+        float s = su1[ci - l * RX] + su1[ci + l * RX];
+        s = __builtin_fmaf(s, m.hzx2_1, su1[ci - l]);
+        s = s + su1[ci + l];
+        w1 = __builtin_fmaf(cr.c(l), s, w1);
+      }
+      const float u0 = U0[(long)gz * P + gx];
+      if (MODE == FR_FWD) {
+        T = combine(u, u0, vj, w1, m, a.f64comb != 0);
+        if (gz == sp.x && gx == sp.y) T = T + a.wav;
+      } else {
+        int s;
+        bool inj = false;
+        if (gz == a.r.s_z && is_receiver(gx, a.r, &s)) {
+          float d = a.data[((long long)shot * a.nt + (a.k + 1)) * a.r.n + s];
+          if (d != 0.0f) { T = d; inj = true; }
+        }
+        if (!inj) T = combine(u, u0, vj, w1, m, false);
+      }
+    }
+    sT[idx] = T;
+  }
+  __syncthreads();
+
+  for (int idx = threadIdx.x; idx < TZ * TX; idx += 256) {
+    const int i = idx / TX, jx = idx - i * TX;
+    const int z = z0 + i, x = x0 + jx;
+    bool in;
+    if (band == 0) in = z < g.n2 && x < g.nx;
+    else if (band == 1) in = z >= g.nz - g.n2 && z < g.nz && x < g.nx;
+    else if (band == 2) in = z >= g.n2 && z < g.nz - g.n2 && x < g.n2;
+    else in = z >= g.n2 && z < g.nz - g.n2 && x >= g.nx - g.n2 && x < g.nx;
+    if (!in) continue;
+    const int ar = min(min(z, x), min(g.nz - 1 - z, g.nx - 1 - x));  // ring
+    const int br = g.nz - 1 - ar, crr = g.nx - 1 - ar;
+    const long long pg = (long)z * P + x;
+    auto Tat = [&](int dz, int dx) { return sT[(i + 1 + dz) * TRX + (jx + 1 + dx)]; };
+    auto U1at = [&](int dz, int dx) { return su1[(i + 1 + NFD + dz) * RX + (jx + 1 + NFD + dx)]; };
+    const float Tp = Tat(0, 0);
+    const bool zc = (z - ar <= 1) || (br - z <= 1);
+    const bool xc = (x - ar <= 1) || (crr - x <= 1);
+    float O;
+    if (zc && xc) {  // corner cells (kernel.cu:138-155)
+      const int dzc = (z - ar <= 1) ? 1 : -1, dxc = (x - ar <= 1) ? 1 : -1;
+      const float rr = R1[pg];
+      O = (1.0f / (2.0f * rr + 1.0f)) * (U1at(0, 0) + rr * (Tat(0, dxc) + Tat(dzc, 0)));
+    } else {
+      int dzi, dxi;        // toward the inner neighbour
+      int lz, lx;          // lateral (+) direction
+      float vq;            // velocity of the taoh2*v^2 term (kernel.cu:124,128,132,136)
+      const float vp = V[pg];
+      if (z == ar) { dzi = 1; dxi = 0; lz = 0; lx = 1; vq = vp; }
+      else if (z == br) { dzi = -1; dxi = 0; lz = 0; lx = 1; vq = V[(long)ar * P + x]; }
+      else {
+        dzi = 0; lz = 1; lx = 0;
+        dxi = (x == ar) ? 1 : -1;
+        const long long lin = (long long)ar * g.nx + z;  // Dv[(N2-y)*NX+x], linear index
+        const int zq = (int)(lin / g.nx), xq = (int)(lin % g.nx);
+        vq = V[(long)zq * P + xq];
+      }
+      const long long pi = pg + (long)dzi * P + dxi;
+      const long long lat = (long)lz * P + lx;
+      const float U0p = U0[pg], U0i = U0[pi];
+      const float Ti = Tat(dzi, dxi);
+      const float Tip = Tat(dzi + lz, dxi + lx), Tim = Tat(dzi - lz, dxi - lx);
+      const float U1p = U1at(0, 0), U1i = U1at(dzi, dxi);
+      const float U0pp = U0[pg + lat], U0pm = U0[pg - lat];
+      const float tv = m.taoh * vp;
+      const float t1 = tv * ((Ti - U0i) + U0p);
+      const float t2 = ((((-2.0f * U1p) + U0p) + Ti) - 2.0f * U1i) + U0i;
+      const float lap = ((((Tip - 2.0f * Ti) + Tim) + U0pp) - 2.0f * U0p) + U0pm;
+      const float t3 = ((m.taoh2 * vq) * vq) * lap;
+      O = (1.0f / (tv + 1.0f)) * ((t1 - t2) + t3);
+    }
+    const float wgt = m.w[g.n2 - ar];
+    const float val = (1.0f - wgt) * Tp + wgt * O;
+    U2[pg] = val;
+    if (MODE == FR_FWD) {
+      if (a.save_strips) {
+        long long e = strip_index(g, z, x);
+        if (e >= 0) strip_put(a.st, shot, a.k, e, val);
+      }
+      int s;
+      if (a.rec && z == a.r.s_z && is_receiver(x, a.r, &s))
+        a.rec[((long long)shot * a.nt + a.k) * a.r.n + s] = val;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small kernels.
+__global__ void k_fill(float* p, long long n, float val) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long st = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = val;
+}
+
+// u(0) = 0, u(1) = f(0)/2 at the source (kernel.cu:798-806).
+__global__ void k_fwd_begin(Grid g, float* f0, float* f1, const int2* src, float spike) {
+  const int shot = blockIdx.y;
+  const long long n = (long long)g.nz * g.pitch;
+  float* a = f0 + (long long)shot * g.fstride;
+  float* b = f1 + (long long)shot * g.fstride;
+  const int2 sp = src[shot];
+  const long long ps = (long long)sp.x * g.pitch + g.x0 + sp.y;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    a[i] = 0.0f;
+    b[i] = (i == ps) ? spike : 0.0f;
+  }
+}
+
+__global__ void k_strip_save(Grid g, const float* f, Strips st, int k) {
+  const int shot = blockIdx.y;
+  const float* F = f + (long long)shot * g.fstride + g.x0;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < st.ss;
+       e += (long long)gridDim.x * blockDim.x) {
+    int z, x;
+    strip_cell(g, e, &z, &x);
+    strip_put(st, shot, k, e, F[(long)z * g.pitch + x]);
+  }
+}
+
+// Quantise the staged fp32 strips of slot k with a power-of-two scale derived
+// from the slot's max |value| (fp16: |value/scale| <= 2^15 < 65504).
+__global__ void k_strip_finish(Grid g, Strips st, int k) {
+  const int shot = blockIdx.y;
+  const long long slot = (long long)shot * st.nslot + (k % st.nslot);
+  const float amax = __uint_as_float(st.amax[slot]);
+  float sc = 1.0f;
+  if (amax > 0.0f) {
+    int e;
+    frexpf(amax, &e);              // amax = f * 2^e, f in [0.5, 1)
+    sc = ldexpf(1.0f, e - 15);     // amax / sc < 2^15
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st.scale[slot] = sc;
+  const float inv = 1.0f / sc;     // exact: sc is a power of two
+  const float* s = st.stage + ((long long)shot * 2 + (k & 1)) * st.ss;
+  unsigned short* d = (unsigned short*)st.base + slot * st.ss;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < st.ss;
+       e += (long long)gridDim.x * blockDim.x) {
+    float x = s[e] * inv;
+    d[e] = (st.fmt == 1) ? __half_as_ushort(__float2half_rn(x)) : f2bf(x);
+  }
+}
+
+__global__ void k_strip_restore(Grid g, float* f, Strips st, int k) {
+  const int shot = blockIdx.y;
+  float* F = f + (long long)shot * g.fstride + g.x0;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < st.ss;
+       e += (long long)gridDim.x * blockDim.x) {
+    int z, x;
+    strip_cell(g, e, &z, &x);
+    F[(long)z * g.pitch + x] = strip_get(st, shot, k, e);
+  }
+}
+
+__global__ void k_record(Grid g, Rcv r, const float* f, float* rec, int k, int nt) {
+  const int shot = blockIdx.y;
+  const float* F = f + (long long)shot * g.fstride + g.x0;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < r.n; s += gridDim.x * blockDim.x)
+    rec[((long long)shot * nt + k) * r.n + s] = F[(long)r.s_z * g.pitch + r.s_l + s * r.ds];
+}
+
+// Backward initial state (kernel.cu:859-885).  s0 = u(nt-1) (DBW0),
+// s1 = u(nt-2) (DBW1); FW0 = 0 and FW1 = spike field are host arrays there.
+__global__ void k_bwd_begin(Grid g, const float* s0, const float* s1, float* q0, float* q1,
+                            float* su, float* sq, float* i1, float* i2, const int2* src,
+                            float spike, int icompen) {
+  const int shot = blockIdx.y;
+  const long long base = (long long)shot * g.fstride;
+  const int2 sp = src[shot];
+  const long long ps = (long long)sp.x * g.pitch + g.x0 + sp.y;
+  const long long n = (long long)g.nz * g.pitch;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long p = base + i;
+    const float BW0 = s0[p], BW1 = s1[p];
+    const float FW1 = (i == ps) ? spike : 0.0f, FW0 = 0.0f;
+    q0[p] = 0.0f;
+    q1[p] = 0.0f;
+    if (icompen == 1) {
+      const float BW2 = BW0 + BW1;
+      const float FW2 = FW0 + FW1;
+      su[p] = BW2;
+      sq[p] = FW2;
+      i1[p] = FW2 * BW2 + FW0 * BW0;
+    } else {
+      su[p] = 0.0f;
+      sq[p] = 0.0f;
+      i1[p] = FW1 * BW1 + FW0 * BW0;
+    }
+    i2[p] = BW1 * BW1 + BW0 * BW0;
+  }
+}
+
+// max |I2| over the interior, per shot (kernel.cu:963-970).
+__global__ void k_maxabs(Grid g, const float* i2, unsigned* out) {
+  const int shot = blockIdx.y;
+  const float* I = i2 + (long long)shot * g.fstride + g.x0;
+  const long long n = (long long)g.mnz * g.mnx;
+  float mx = 0.0f;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    int z = g.n2 + (int)(e / g.mnx), x = g.n2 + (int)(e % g.mnx);
+    mx = fmaxf(mx, fabsf(I[(long)z * g.pitch + x]));
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(&out[shot], __float_as_uint(mx));
+}
+
+// Per-shot image post (kernel.cu:935-979).
+__global__ void k_post(Grid g, const float* v, const float* i1, const float* i2,
+                       const unsigned* amax, float vmax, float whitecoe, float* up, float* down) {
+  const int shot = blockIdx.y;
+  const float* M = i1 + (long long)shot * g.fstride + g.x0;
+  const float* I = i2 + (long long)shot * g.fstride + g.x0;
+  const float* Vv = v + g.x0;
+  const long long n = (long long)g.mnz * g.mnx;
+  const float stable = __uint_as_float(amax[shot]) * whitecoe;
+  const int P = g.pitch, n2 = g.n2;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int i = n2 + (int)(e / g.mnx), j = n2 + (int)(e % g.mnx);
+    int i1_ = i - 1, i2_ = i + 1, j1 = j - 1, j2 = j + 1;
+    if (i1_ < n2) i1_ = 2 * n2 - i1_;
+    if (j1 < n2) j1 = 2 * n2 - j1;
+    if (i2_ >= g.nz - n2) i2_ = 2 * (g.nz - n2 - 1) - i2_;
+    if (j2 >= g.nx - n2) j2 = 2 * (g.nx - n2 - 1) - j2;
+    const float sum = M[(long)i * P + j2] + M[(long)i * P + j1] + M[(long)i2_ * P + j] +
+                      M[(long)i1_ * P + j] - 4.0f * M[(long)i * P + j];
+    const float vv = Vv[(long)i * P + j];
+    const double t = -1.0 * (double)sum * (double)vv * (double)vv / (double)(vmax * vmax);
+    const long long o = (long long)shot * n + e;
+    if (up) up[o] = (float)t;
+    if (down) down[o] = I[(long)i * P + j] + stable;
+  }
+}
+
+// Running stack in shot order (kernel.cu:1026-1039).
+__global__ void k_stack_add(int nI, const float* up, const float* down, float* s1, float* s2,
+                            int nshot) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nI; e += gridDim.x * blockDim.x) {
+    float a = s1[e], b = s2[e];
+    for (int s = 0; s < nshot; ++s) {
+      a += up[(long long)s * nI + e];
+      b += down[(long long)s * nI + e];
+    }
+    s1[e] = a;
+    s2[e] = b;
+  }
+}
+
+__global__ void k_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
+                               float* img, float* ill) {  // kernel.cu:1042-1059
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nI; e += gridDim.x * blockDim.x) {
+    float a = s1[e] / nrec;
+    float b = s2[e] / nrec;
+    if (inorm == 1) a = a / b;
+    img[e] = a;
+    ill[e] = b;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+#define RTM_NFD_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+
+hipError_t launch_interior(int epi, int nfd, const IntArgs& a, int nshot, int nblk, hipStream_t s) {
+  dim3 grid(nblk, nshot), block(256);
+#define CASE(N)                                                                          \
+  case N:                                                                                \
+    if (epi == EPI_FWD) hipLaunchKernelGGL((k_interior<N, EPI_FWD>), grid, block, 0, s, a); \
+    else if (epi == EPI_SRC) hipLaunchKernelGGL((k_interior<N, EPI_SRC>), grid, block, 0, s, a); \
+    else hipLaunchKernelGGL((k_interior<N, EPI_RCV>), grid, block, 0, s, a);            \
+    break;
+  switch (nfd) {
+    RTM_NFD_CASES(CASE)
+    default: return hipErrorInvalidValue;
+  }
+#undef CASE
+  return hipGetLastError();
+}
+
+hipError_t launch_frame(int mode, int nfd, const FrameArgs& a, int nshot, int ntiles, hipStream_t s) {
+  dim3 grid(ntiles, nshot), block(256);
+#define CASE(N)                                                                          \
+  case N:                                                                                \
+    if (mode == FR_FWD) hipLaunchKernelGGL((k_frame<N, FR_FWD>), grid, block, 0, s, a);  \
+    else hipLaunchKernelGGL((k_frame<N, FR_RCV>), grid, block, 0, s, a);                 \
+    break;
+  switch (nfd) {
+    RTM_NFD_CASES(CASE)
+    default: return hipErrorInvalidValue;
+  }
+#undef CASE
+  return hipGetLastError();
+}
+
+static inline unsigned blocks_for(long long n, int per = 256, unsigned cap = 4096) {
+  long long b = (n + per - 1) / per;
+  if (b < 1) b = 1;
+  return (unsigned)(b > cap ? cap : b);
+}
+
+hipError_t launch_fill(float* p, long long n, float val, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, s, p, n, val);
+  return hipGetLastError();
+}
+hipError_t launch_fwd_begin(const Grid& g, float* f0, float* f1, const int2* src, float spike,
+                            int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_fwd_begin, dim3(blocks_for((long long)g.nz * g.pitch, 256, 1024), nshot),
+                     dim3(256), 0, s, g, f0, f1, src, spike);
+  return hipGetLastError();
+}
+hipError_t launch_strip_save(const Grid& g, const float* f, Strips st, int k, int nshot,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_strip_save, dim3(blocks_for(st.ss), nshot), dim3(256), 0, s, g, f, st, k);
+  return hipGetLastError();
+}
+hipError_t launch_strip_finish(const Grid& g, Strips st, int k, int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_strip_finish, dim3(blocks_for(st.ss), nshot), dim3(256), 0, s, g, st, k);
+  return hipGetLastError();
+}
+hipError_t launch_strip_restore(const Grid& g, float* f, Strips st, int k, int nshot,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_strip_restore, dim3(blocks_for(st.ss), nshot), dim3(256), 0, s, g, f, st, k);
+  return hipGetLastError();
+}
+hipError_t launch_record(const Grid& g, const Rcv& r, const float* f, float* rec, int k, int nt,
+                         int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_record, dim3(blocks_for(r.n), nshot), dim3(256), 0, s, g, r, f, rec, k, nt);
+  return hipGetLastError();
+}
+hipError_t launch_bwd_begin(const Grid& g, const float* s0, const float* s1, float* q0, float* q1,
+                            float* su, float* sq, float* i1, float* i2, const int2* src,
+                            float spike, int icompen, int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_bwd_begin, dim3(blocks_for((long long)g.nz * g.pitch, 256, 1024), nshot),
+                     dim3(256), 0, s, g, s0, s1, q0, q1, su, sq, i1, i2, src, spike, icompen);
+  return hipGetLastError();
+}
+hipError_t launch_maxabs(const Grid& g, const float* i2, unsigned* out, int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_maxabs, dim3(blocks_for((long long)g.mnz * g.mnx, 256, 256), nshot),
+                     dim3(256), 0, s, g, i2, out);
+  return hipGetLastError();
+}
+hipError_t launch_post(const Grid& g, const float* v, const float* i1, const float* i2,
+                       const unsigned* amax, float vmax, float whitecoe, float* up, float* down,
+                       int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_post, dim3(blocks_for((long long)g.mnz * g.mnx, 256, 1024), nshot),
+                     dim3(256), 0, s, g, v, i1, i2, amax, vmax, whitecoe, up, down);
+  return hipGetLastError();
+}
+hipError_t launch_stack_add(int nI, const float* up, const float* down, float* s1, float* s2,
+                            int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_stack_add, dim3(blocks_for(nI)), dim3(256), 0, s, nI, up, down, s1, s2, nshot);
+  return hipGetLastError();
+}
+hipError_t launch_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
+                               float* img, float* illum, hipStream_t s) {
+  hipLaunchKernelGGL(k_stack_finish, dim3(blocks_for(nI)), dim3(256), 0, s, nI, s1, s2, nrec, inorm,
+                     img, illum);
+  return hipGetLastError();
+}
+
+}  // namespace rtm

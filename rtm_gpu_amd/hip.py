@@ -1,0 +1,193 @@
+"""ctypes binding of librtm_hip.so (include/rtm_hip.h).
+
+Loading fails loudly if the library has not been built: there is no CPU
+fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "librtm_hip.so")
+
+RTM_STRIP_F32, RTM_STRIP_F16, RTM_STRIP_BF16 = 0, 1, 2
+FIELD_U, FIELD_U_PREV, FIELD_Q, FIELD_I1, FIELD_I2, FIELD_SU, FIELD_SQ = range(7)
+K_FWD_INTERIOR, K_FWD_FRAME, K_SRC_INTERIOR, K_STRIP_RESTORE, K_RCV_INTERIOR, K_RCV_FRAME = range(6)
+KERNEL_NAMES = {
+    K_FWD_INTERIOR: "k_interior<EPI_FWD>", K_FWD_FRAME: "k_frame<FR_FWD>",
+    K_SRC_INTERIOR: "k_interior<EPI_SRC>", K_STRIP_RESTORE: "k_strip_restore",
+    K_RCV_INTERIOR: "k_interior<EPI_RCV>", K_RCV_FRAME: "k_frame<FR_RCV>",
+}
+
+
+class RtmParams(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nz", "nx", "n2", "nfdmax", "nt", "nt2", "ilste",
+                                        "icompen", "s_l", "s_r", "s_z", "ds", "n")] + \
+               [(n, C.c_float) for n in ("tao2", "h2", "taoh", "taoh2", "hzx2_1")] + \
+               [(n, C.c_int) for n in ("max_shots", "strip_format", "record", "zseg")]
+
+
+class RtmError(RuntimeError):
+    pass
+
+
+_lib = None
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RtmError(f"{LIB_PATH} is missing: run `python -m rtm_gpu_amd.build` first")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.rtm_create.argtypes = [C.POINTER(RtmParams), C.POINTER(vp)]
+        L.rtm_destroy.argtypes = [vp]
+        L.rtm_last_error.argtypes = [vp]
+        L.rtm_last_error.restype = C.c_char_p
+        L.rtm_device_bytes.argtypes = [vp]
+        L.rtm_device_bytes.restype = C.c_size_t
+        L.rtm_set_model.argtypes = [vp, _fp, _fp, _fp, _fp, C.c_int, vp, C.c_int, C.c_float,
+                                    C.c_float]
+        L.rtm_set_wavelet.argtypes = [vp, _fp]
+        L.rtm_set_shots.argtypes = [vp, C.c_int, _ip, _ip]
+        L.rtm_set_gathers.argtypes = [vp, _fp]
+        L.rtm_forward_begin.argtypes = [vp, C.c_int]
+        L.rtm_forward_advance.argtypes = [vp, C.c_int]
+        L.rtm_backward_begin.argtypes = [vp]
+        L.rtm_backward_advance.argtypes = [vp, C.c_int]
+        L.rtm_migrate.argtypes = [vp]
+        L.rtm_sync.argtypes = [vp]
+        L.rtm_read_field.argtypes = [vp, C.c_int, C.c_int, _fp]
+        L.rtm_read_records.argtypes = [vp, _fp]
+        L.rtm_read_strip.argtypes = [vp, C.c_int, C.c_int, _fp]
+        L.rtm_image_post.argtypes = [vp, C.c_float, C.c_float, vp, vp]
+        L.rtm_stack_finish.argtypes = [vp, C.c_int, C.c_int, _fp, vp]
+        L.rtm_set_timing.argtypes = [vp, C.c_int]
+        L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+        for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
+                  "rtm_set_gathers", "rtm_forward_begin", "rtm_forward_advance",
+                  "rtm_backward_begin", "rtm_backward_advance", "rtm_migrate", "rtm_sync",
+                  "rtm_read_field", "rtm_read_records", "rtm_read_strip", "rtm_image_post",
+                  "rtm_stack_finish", "rtm_set_timing", "rtm_kernel_time"):
+            getattr(L, n).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+class Context:
+    """One librtm_hip context (one HIP stream, one batch of shots)."""
+
+    def __init__(self, **params):
+        p = RtmParams(**{k: params.get(k, 0) for k, _ in RtmParams._fields_})
+        if p.max_shots == 0:
+            p.max_shots = 1
+        self.p = p
+        h = C.c_void_p()
+        rc = lib().rtm_create(C.byref(p), C.byref(h))
+        if rc != 0:
+            raise RtmError(f"rtm_create failed ({rc})")
+        self.h = h
+        self.nz, self.nx, self.n2 = p.nz, p.nx, p.n2
+        self.nshot = 0
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = lib().rtm_last_error(self.h).decode()
+            raise RtmError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rtm_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def device_bytes(self):
+        return lib().rtm_device_bytes(self.h)
+
+    def set_model(self, v, r1, w, c, index, nvel, vmin, dv):
+        self._keep_model = [np.ascontiguousarray(a, np.float32).ravel() for a in (v, r1, w, c)]
+        idx = None
+        if index is not None:
+            self._idx = np.ascontiguousarray(index, np.int32)
+            idx = self._idx.ctypes.data
+        self._chk(lib().rtm_set_model(self.h, *self._keep_model, len(self._keep_model[3]), idx,
+                                      int(nvel), float(vmin), float(dv)), "rtm_set_model")
+
+    def set_wavelet(self, wav):
+        self._chk(lib().rtm_set_wavelet(self.h, np.ascontiguousarray(wav, np.float32)),
+                  "rtm_set_wavelet")
+
+    def set_shots(self, r_u, r_x):
+        r_u = np.ascontiguousarray(r_u, np.int32)
+        r_x = np.ascontiguousarray(r_x, np.int32)
+        self._chk(lib().rtm_set_shots(self.h, len(r_u), r_u, r_x), "rtm_set_shots")
+        self.nshot = len(r_u)
+
+    def set_gathers(self, seis):
+        self._chk(lib().rtm_set_gathers(self.h, np.ascontiguousarray(seis, np.float32).ravel()),
+                  "rtm_set_gathers")
+
+    def forward_begin(self, save_strips=True):
+        self._chk(lib().rtm_forward_begin(self.h, int(bool(save_strips))), "rtm_forward_begin")
+
+    def forward_advance(self, n):
+        self._chk(lib().rtm_forward_advance(self.h, int(n)), "rtm_forward_advance")
+
+    def backward_begin(self):
+        self._chk(lib().rtm_backward_begin(self.h), "rtm_backward_begin")
+
+    def backward_advance(self, n):
+        self._chk(lib().rtm_backward_advance(self.h, int(n)), "rtm_backward_advance")
+
+    def migrate(self):
+        self._chk(lib().rtm_migrate(self.h), "rtm_migrate")
+
+    def sync(self):
+        self._chk(lib().rtm_sync(self.h), "rtm_sync")
+
+    def field(self, which, shot=0):
+        out = np.zeros(self.nz * self.nx, np.float32)
+        self._chk(lib().rtm_read_field(self.h, shot, which, out), "rtm_read_field")
+        return out.reshape(self.nz, self.nx)
+
+    def records(self):
+        out = np.zeros(self.nshot * self.p.n * self.p.nt, np.float32)
+        self._chk(lib().rtm_read_records(self.h, out), "rtm_read_records")
+        return out.reshape(self.nshot, self.p.n, self.p.nt)
+
+    def strip(self, shot, k):
+        ss = 2 * self.p.nfdmax * (self.nz - 2 * self.n2 + self.nx - 2 * self.n2)
+        out = np.zeros(ss, np.float32)
+        self._chk(lib().rtm_read_strip(self.h, shot, k, out), "rtm_read_strip")
+        return out
+
+    def image_post(self, vmax, whitecoe, want=True):
+        mnz, mnx = self.nz - 2 * self.n2, self.nx - 2 * self.n2
+        up = np.zeros((self.nshot, mnz, mnx), np.float32) if want else None
+        down = np.zeros((self.nshot, mnz, mnx), np.float32) if want else None
+        self._chk(lib().rtm_image_post(self.h, float(vmax), float(whitecoe),
+                                       up.ctypes.data if want else None,
+                                       down.ctypes.data if want else None), "rtm_image_post")
+        return up, down
+
+    def stack_finish(self, nrec, inorm):
+        mnz, mnx = self.nz - 2 * self.n2, self.nx - 2 * self.n2
+        img = np.zeros(mnz * mnx, np.float32)
+        ill = np.zeros(mnz * mnx, np.float32)
+        self._chk(lib().rtm_stack_finish(self.h, int(nrec), int(inorm), img, ill.ctypes.data),
+                  "rtm_stack_finish")
+        return img.reshape(mnz, mnx), ill.reshape(mnz, mnx)
+
+    def set_timing(self, on):
+        self._chk(lib().rtm_set_timing(self.h, int(bool(on))), "rtm_set_timing")
+
+    def kernel_time(self, kind):
+        ms = C.c_double()
+        n = C.c_longlong()
+        self._chk(lib().rtm_kernel_time(self.h, kind, C.byref(ms), C.byref(n)), "rtm_kernel_time")
+        return ms.value, n.value

@@ -1,0 +1,171 @@
+"""GPU parity: librtm_hip.so (called through its C ABI) against the FP32
+replica in oracle/ on the same seeded inputs.
+
+The bar: per-100-step wavefield relative L2 <= 1e-5 and final image relative
+L2 <= 1e-4 against the FP32 replica.  The kernels evaluate the same operation
+sequence as the replica, so these tests also report (and require) exact
+equality of the values; the FP64 oracle error is printed for information."""
+import numpy as np
+import pytest
+
+import cases
+from cases import build_case, hip_context, oracle_engine, rel_l2, synth_gather
+
+pytestmark = pytest.mark.gpu
+
+WAVE_TOL = 1e-5
+IMAGE_TOL = 1e-4
+
+CASES = {
+    # adaptive LS coefficients, compensated imaging, source inside the interior
+    "ls_nfd4": dict(name="ls_nfd4"),
+    # Taylor coefficients (Add_Con / BKAdd_EFF_Con / BKAdd_Con), non-compensated
+    "taylor_nfd6": dict(name="taylor_nfd6", ilste=1, nfdmax=6, icompen=0, seed=2),
+    # source in the ABC frame (row N2-1, as the first cfg2 source) and a
+    # receiver line with ds=3 reaching into the right frame
+    "frame_src": dict(name="frame_src", src_depth=5.0, ds=3, s_l1=4, n=26, nfdmax=5, n2=8,
+                      seed=3, mod_nx=72),
+    # maximum operator length of the reference parameter file (nfdmax = N2 = 10)
+    "ls_nfd10_n2_10": dict(name="ls_nfd10_n2_10", nfdmax=10, n2=10, mod_nz=36, mod_nx=48, seed=4,
+                           h=12.5, hz=12.5, f0=20.0, vhi=3500.0),
+}
+
+
+def _case(key):
+    return build_case(**CASES[key])
+
+
+def _fwd_compare(case, every=40):
+    ctx = hip_context(case, nshot=1)
+    eng = oracle_engine(case)
+    nt = case["cfg"]["nt"]
+    ctx.forward_begin()
+    eng.forward_begin()
+    k = 2
+    worst = 0.0
+    while k < nt:
+        step = min(every, nt - k)
+        ctx.forward_advance(step)
+        eng.forward_advance(step)
+        k += step
+        g = ctx.field(cases_FIELD_U())
+        o = eng.field(0)
+        err = rel_l2(g, o)
+        worst = max(worst, err)
+        assert err <= WAVE_TOL, f"step {k}: rel L2 {err}"
+        assert np.array_equal(g, o), f"step {k}: max |diff| {np.max(np.abs(g - o))}"
+    return ctx, eng, worst
+
+
+def cases_FIELD_U():
+    from rtm_gpu_amd import hip
+    return hip.FIELD_U
+
+
+@pytest.mark.parametrize("key", list(CASES))
+def test_forward_wavefield_parity(key):
+    case = _case(key)
+    print(f"case {key}: grid {case['cfg']['nz']}x{case['cfg']['nx']} nt={case['cfg']['nt']}",
+          flush=True)
+    ctx, eng, worst = _fwd_compare(case)
+    print(f"  forward worst rel L2 vs FP32 replica: {worst:.3e}")
+    # strips of a few slots
+    for k in (0, 1, 2, case["cfg"]["nt"] // 2, case["cfg"]["nt"] - 1):
+        assert np.array_equal(ctx.strip(0, k), eng.strip(k)), f"strip slot {k}"
+
+
+@pytest.mark.parametrize("key", ["ls_nfd4", "frame_src"])
+def test_receiver_records(key):
+    from rtm_gpu_amd import hip  # noqa: F401
+    case = _case(key)
+    ctx = hip_context(case, nshot=1, record=1)
+    ctx.forward_begin(save_strips=False)
+    ctx.forward_advance(case["cfg"]["nt"])
+    rec = ctx.records()[0]
+    ref = synth_gather(case)
+    assert rec.shape == ref.shape
+    assert np.array_equal(rec, ref)
+    assert np.abs(ref).max() > 0
+
+
+@pytest.mark.parametrize("key", list(CASES))
+def test_backward_and_images(key):
+    from rtm_gpu_amd import hip
+    case = _case(key)
+    seis = synth_gather(case)
+    ctx = hip_context(case, nshot=1)
+    ctx.set_gathers(seis[None])
+    eng = oracle_engine(case)
+    nt = case["cfg"]["nt"]
+    ctx.forward_begin()
+    ctx.forward_advance(nt)
+    eng.forward_begin()
+    eng.forward_advance(nt)
+    ctx.backward_begin()
+    eng.backward_begin(seis)
+    done = 0
+    while done < nt - 2:
+        step = min(50, nt - 2 - done)
+        ctx.backward_advance(step)
+        eng.backward_advance(step)
+        done += step
+        for which, ow in ((hip.FIELD_Q, 2), (hip.FIELD_I1, 3), (hip.FIELD_I2, 4)):
+            g = ctx.field(which)
+            o = eng.field(ow)
+            err = rel_l2(g, o)
+            assert err <= WAVE_TOL, f"field {which} after {done} backward steps: {err}"
+            assert np.array_equal(g, o), f"field {which} after {done}: {np.max(np.abs(g - o))}"
+        # the reconstructed source field on the interior
+        n2 = case["cfg"]["n2"]
+        g = ctx.field(hip.FIELD_U)[n2:-n2, n2:-n2]
+        o = eng.field(0)[n2:-n2, n2:-n2]
+        assert np.array_equal(g, o), f"u after {done}"
+    i1 = ctx.field(hip.FIELD_I1)
+    o1 = eng.field(3)
+    assert rel_l2(i1, o1) <= IMAGE_TOL
+    e64 = oracle_engine(case, prec=64)
+    e64.forward_begin()
+    e64.forward_advance(nt)
+    e64.backward_begin(seis)
+    e64.backward_advance(nt)
+    print(f"case {key}: image rel L2 vs FP32 replica {rel_l2(i1, o1):.2e}, "
+          f"vs FP64 oracle {rel_l2(i1, e64.field(3, f64=True)):.2e}")
+
+
+def test_batched_shots_equal_single():
+    case = build_case(name="batch3", nshot=3, depth_step=40.0)
+    seis = np.stack([synth_gather(case, s) for s in range(3)])
+    ctx = hip_context(case, nshot=3)
+    ctx.set_gathers(seis)
+    ctx.migrate()
+    from rtm_gpu_amd import hip
+    for s in range(3):
+        eng = oracle_engine(case, shot=s)
+        eng.forward_begin()
+        eng.forward_advance(case["cfg"]["nt"])
+        eng.backward_begin(seis[s])
+        eng.backward_advance(case["cfg"]["nt"])
+        assert np.array_equal(ctx.field(hip.FIELD_I1, shot=s), eng.field(3)), f"shot {s}"
+        assert np.array_equal(ctx.field(hip.FIELD_I2, shot=s), eng.field(4)), f"shot {s}"
+
+
+def test_image_post_and_stack():
+    import oracle_bind as ob
+    case = build_case(name="batch3", nshot=3, depth_step=40.0)
+    seis = np.stack([synth_gather(case, s) for s in range(3)])
+    ctx = hip_context(case, nshot=3)
+    ctx.set_gathers(seis)
+    ctx.migrate()
+    m = case["model"]
+    up, down = ctx.image_post(m["vmax"], 1e-4)
+    img, ill = ctx.stack_finish(3, 1)
+    from rtm_gpu_amd import hip
+    ups, downs = [], []
+    for s in range(3):
+        u, d = ob.image_post(ctx.field(hip.FIELD_I1, s), ctx.field(hip.FIELD_I2, s), m["v"],
+                             case["cfg"]["n2"], m["vmax"], 1e-4)
+        assert np.array_equal(u, up[s]) and np.array_equal(d, down[s]), f"post shot {s}"
+        ups.append(u)
+        downs.append(d)
+    oimg, oill = ob.stack(np.stack(ups), np.stack(downs), 1)
+    assert np.array_equal(img, oimg) and np.array_equal(ill, oill)
