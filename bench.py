@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Headline benchmark: 2-D acoustic reverse-time migration on one MI355X.
+
+Workload (the reverse-VSP configuration): 351 x 1701 interior grid at
+12.5 m (411 x 1761 with the 30-point hybrid ABC), seeded faulted-layer model
+with 8 dipping layers (1500-5500 m/s), 3 normal faults and a 1200 m x 600 m
+4500 m/s body; 128 sources down a well at x = 850 (6.25 m apart), 1701
+surface receivers, 20 Hz Ricker, dt = 1 ms, 4000 steps, adaptive LS
+coefficients (nfdmax 10), compensated imaging, image stacked over shots.
+
+Observed data are modelled on the true model by the same engine (forward
+pass with receiver recording, untimed); migration uses a smoothed model.
+One step = forward pass + backward pass with imaging + per-shot image post
+and stack for the whole batch of sources, with inputs resident on the GPU.
+
+  python bench.py [--steps K] [--warmup W] [--shots S] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Gpt-updates/s (%HBM peak), shots/s; time-to-5%-image-err: CNN vs plain LSRTM")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_problem(shots, nfdmax=10, n2=30, nt=4000):
+    from rtm_gpu_amd import host, synth
+    mod_nz, mod_nx, h, tao, f0 = 351, 1701, 12.5, 0.001, 20.0
+    cfg = host.make_config(nfdmax=nfdmax, nfdmin=2, n2=n2, f0=f0, fmax=2 * f0 + 1, df=1.0,
+                           nthita=1000, eps=1e-5, dv=1.0, ilste=0, ifv=0, whitecoe=1e-4, hz=h,
+                           tao=tao, inorm=1, icompen=1, h=h, tao1=tao, mod_nz=mod_nz,
+                           mod_nx=mod_nx, nt1=nt, s_l1=1, s_z1=1, n=mod_nx, ds=1, r_x1=851,
+                           nrec=shots, dr=1)
+    d = host.derive(cfg)
+    true_raw = synth.faulted_rvsp(seed=2, mod_nz=mod_nz, mod_nx=mod_nx, h=h)
+    mig_raw = synth.smooth(true_raw, 4.0)
+    models = {}
+    for name, raw in (("true", true_raw), ("mig", mig_raw)):
+        t = time.time()
+        v, r1 = host.pad_model(raw, mod_nz, mod_nx, n2, 0, tao, h)
+        vmin, vmax, nvel = host.vrange(v, 1.0)
+        need = host.bins_needed(v, vmin, 1.0, nvel)
+        M, index, c = host.design_ls(1000, nfdmax, 2, nvel, tao, h, 1.0, 1e-5, cfg.fmax, vmin,
+                                     1.0, need, d.hzx, nthreads=min(16, os.cpu_count() or 1))
+        models[name] = dict(v=v, r1=r1, c=c, index=index, nvel=nvel, vmin=vmin, vmax=vmax,
+                            nbins=int(need.sum()), design_s=time.time() - t)
+        log(f"[bench] {name} model: {int(need.sum())} velocity bins, M in "
+            f"{sorted(set(M[M > 0].tolist()))}, design {time.time() - t:.1f}s")
+    depths = [6.25 * (m + 1) for m in range(shots)]
+    r_u = [host.source_row(z, h, n2) for z in depths]
+    r_x = [d.r_x] * shots
+    wav = host.wavelet_table(d.nt, tao, f0)
+    w = host.blend_weights(n2)
+    return dict(cfg=cfg, d=d, models=models, r_u=r_u, r_x=r_x, wav=wav, w=w)
+
+
+def algorithmic_bytes(kind, d, n2, nfd, nshot, n):
+    """Minimum HBM bytes one launch of each kernel must move.  Per interior
+    cell and shot: forward / source reconstruction read u1, u0 and write u2
+    (12 B); the receiver step reads q1, q0, u(k), sum u, sum q, I1, I2 and
+    writes q2, sum u, sum q, I1, I2 (48 B).  The velocity model is shared by
+    the shots of a launch (4 B per cell per launch)."""
+    from rtm_gpu_amd import hip
+    nz, nx = d.nz, d.nx
+    nI = (nz - 2 * n2) * (nx - 2 * n2)
+    nF = nz * nx - nI
+    ss = 2 * nfd * ((nz - 2 * n2) + (nx - 2 * n2))
+    if kind == hip.K_FWD_INTERIOR or kind == hip.K_SRC_INTERIOR:
+        return nshot * nI * 12 + nI * 4
+    if kind == hip.K_FWD_FRAME:
+        return nshot * (nF * 12 + ss * 4) + nF * 4
+    if kind == hip.K_STRIP_RESTORE:
+        return nshot * ss * 8
+    if kind == hip.K_RCV_INTERIOR:
+        return nshot * (nI * 48 + n * 4) + nI * 4
+    if kind == hip.K_RCV_FRAME:
+        return nshot * (nF * 12 + n * 4) + nF * 4
+    return 0
+
+
+def pmc_traffic(kind_name, workload):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary, if any."""
+    p = os.path.join(ROOT, "profiles", "pmc_summary.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            s = json.load(f)
+        e = s.get(workload, {}).get(kind_name)
+        if not e:
+            return None
+        return float(e["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def cpu_baseline(P, target_s=15.0):
+    """The FP32 CPU restatement (oracle/) on the host cores: one shot per
+    pthread, a bounded number of forward and backward steps on the full grid."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes as C
+    import oracle_bind as ob
+    d, m, cfgp = P["d"], P["models"]["mig"], P["cfg"]
+    n2 = cfgp.n2
+    cores = max(1, min(16, os.cpu_count() or 1))
+    cfg = ob.OrcCfg(nz=d.nz, nx=d.nx, n2=n2, nfdmax=cfgp.nfdmax, nt=d.nt, nt2=d.nt2, ilste=0,
+                    icompen=1, s_l=d.s_l, s_r=d.s_r, s_z=d.s_z, ds=1, n=cfgp.n,
+                    r_u=P["r_u"][len(P["r_u"]) // 2], r_x=d.r_x, tao2=d.tao2, h2=d.h2,
+                    taoh=d.taoh, taoh2=d.taoh2, hzx2_1=d.hzx2_1, vmin=m["vmin"], dv=1.0,
+                    record=0, nthreads=1, strip_slots=64)
+    seis = np.zeros(cfgp.n * d.nt, np.float32)
+    args = [m["v"].ravel(), m["r1"].ravel(), P["w"], m["c"], len(m["c"]),
+            m["index"].ctypes.data, m["nvel"], P["wav"], seis]
+    ob.lib()
+    probe_steps = 3
+    t_probe = ob.lib().orc_time_shots(C.byref(cfg), *args, 1, 1, probe_steps, probe_steps)
+    per_pair = max(t_probe / probe_steps, 1e-3)
+    steps = int(max(4, min(d.nt - 3, target_s / per_pair)))
+    el = ob.lib().orc_time_shots(C.byref(cfg), *args, cores, cores, steps, steps)
+    nI = (d.nz - 2 * n2) * (d.nx - 2 * n2)
+    upd = cores * steps * (2 * d.nz * d.nx + nI)
+    return {"value": upd / el / 1e9, "unit": "Gpt-updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP32 restatement (oracle/), {cores} shots on {cores} pthreads, "
+                      f"{steps} forward + {steps} backward steps each on the {d.nz}x{d.nx} "
+                      f"grid, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=128)
+    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--strips", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--zseg", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {a.gpus} requested); shots shard across "
+              "GPUs with one image all-reduce at the end, which this build does not run.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from rtm_gpu_amd import hip
+    t_setup = time.time()
+    P = build_problem(a.shots, nt=a.nt)
+    d, cfg = P["d"], P["cfg"]
+    strip_fmt = {"f32": 0, "f16": 1, "bf16": 2}[a.strips]
+    ctx = hip.Context(nz=d.nz, nx=d.nx, n2=cfg.n2, nfdmax=cfg.nfdmax, nt=d.nt, nt2=d.nt2,
+                      ilste=0, icompen=1, s_l=d.s_l, s_r=d.s_r, s_z=d.s_z, ds=1, n=cfg.n,
+                      tao2=d.tao2, h2=d.h2, taoh=d.taoh, taoh2=d.taoh2, hzx2_1=d.hzx2_1,
+                      max_shots=a.shots, strip_format=strip_fmt, record=1, zseg=a.zseg)
+    log(f"[bench] context: {ctx.device_bytes() / 2**30:.1f} GiB on device")
+    mt, mm = P["models"]["true"], P["models"]["mig"]
+    ctx.set_wavelet(P["wav"])
+    ctx.set_shots(P["r_u"], P["r_x"])
+    # observed data: forward modelling with receiver recording on the true model
+    ctx.set_model(mt["v"], mt["r1"], P["w"], mt["c"], mt["index"], mt["nvel"], mt["vmin"], 1.0)
+    ctx.forward_begin(save_strips=False)
+    ctx.forward_advance(d.nt)
+    ctx.gathers_from_records()
+    ctx.sync()
+    ctx.set_model(mm["v"], mm["r1"], P["w"], mm["c"], mm["index"], mm["nvel"], mm["vmin"], 1.0)
+    log(f"[bench] setup + modelling {time.time() - t_setup:.1f}s")
+
+    def step():
+        ctx.migrate()
+        ctx.image_post(mm["vmax"], cfg.whitecoe, want=False)
+
+    for i in range(a.warmup):
+        t = time.time()
+        step()
+        ctx.sync()
+        log(f"[bench] warmup {i + 1}/{a.warmup}: {time.time() - t:.2f}s")
+    ctx.set_timing(-1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        step()
+        ctx.sync()
+        log(f"[bench] step {i + 1}/{a.steps} done at {time.perf_counter() - t0:.2f}s")
+    ctx.sync()
+    el = time.perf_counter() - t0
+    img, _ = ctx.stack_finish(a.shots * (a.warmup + a.steps), 1)
+    assert np.isfinite(img).all(), "non-finite image"
+
+    nI = (d.nz - 2 * cfg.n2) * (d.nx - 2 * cfg.n2)
+    steps_f = d.nt - 2
+    upd_per_shot = steps_f * d.nz * d.nx + steps_f * (nI + d.nz * d.nx)
+    value = upd_per_shot * a.shots * a.steps / el / 1e9
+    kinds = {}
+    for k, name in hip.KERNEL_NAMES.items():
+        ms, n = ctx.kernel_time(k)
+        if n:
+            b = algorithmic_bytes(k, d, cfg.n2, cfg.nfdmax, a.shots, cfg.n)
+            kinds[name] = dict(total_ms=ms, launches=n, avg_ms=ms / n,
+                               gbs=b / (ms / n * 1e-3) / 1e9, bytes_per_launch=b)
+    dom = max(kinds, key=lambda k: kinds[k]["total_ms"])
+    kd = kinds[dom]
+    workload = f"rvsp_faulted_351x1701_nt{d.nt}_{a.shots}shots"
+    total_bytes = sum(v["bytes_per_launch"] * v["launches"] for v in kinds.values())
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gpt-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(el / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded faulted-layer model; gathers modelled by the same engine)",
+        "config": {"workload": workload, "grid": [d.nz, d.nx], "interior": [d.nz - 60, d.nx - 60],
+                   "nt": d.nt, "shots": a.shots, "nfdmax": cfg.nfdmax, "n2": cfg.n2,
+                   "strips": a.strips, "coefficients": "adaptive LS",
+                   "velocity_bins": mm["nbins"]},
+        "shots_per_s": round(a.shots * a.steps / el, 3),
+        "hbm_frac_whole_job": round(total_bytes / el / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(kd["gbs"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kd["gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": pmc_traffic(dom, workload),
+                     "algorithmic_bytes_per_launch": kd["bytes_per_launch"],
+                     "avg_launch_ms": round(kd["avg_ms"], 5)},
+        "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "launches": v["launches"],
+                        "GB/s": round(v["gbs"], 1)} for k, v in kinds.items()},
+    }
+    ctx.close()
+    if not a.no_cpu:
+        log("[bench] CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline(P)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -94,6 +94,12 @@ int rtm_set_shots(rtm_ctx* ctx, int nshot, const int* r_u, const int* r_x);
  * kernel.cu:905-909). */
 int rtm_set_gathers(rtm_ctx* ctx, const float* seis);
 
+/* Use the receiver records of the last forward pass (rtm_forward_begin with a
+ * context created with record = 1) as the observed gathers of the batch,
+ * without a host round trip.  The reference reads gathers from disk instead
+ * (kernel.cu:827-838); this serves forward modelling followed by migration. */
+int rtm_gathers_from_records(rtm_ctx* ctx);
+
 /* Forward pass (kernel.cu:798-821): begin sets u(0)=0, u(1)=f(0)/2 at the
  * source and strip slots 0,1; advance runs up to nsteps of k = 2..nt-1.
  * save_strips = 0 skips the boundary strips (pure modelling). */
@@ -124,8 +130,9 @@ int rtm_image_post(rtm_ctx* ctx, float vmax, float whitecoe, float* up, float* d
  * img /= illum when inorm = 1.  Interior [mnz][mnx].  Resets the stack. */
 int rtm_stack_finish(rtm_ctx* ctx, int nrec, int inorm, float* img, float* illum);
 
-/* Kernel timing with HIP events on the context stream (0 = off). */
-int rtm_set_timing(rtm_ctx* ctx, int on);
+/* Kernel timing with HIP events on the context stream: mask is a bit set of
+ * kernel kinds (1 << RTM_K_*), 0 = off, -1 = all.  Resets the counters. */
+int rtm_set_timing(rtm_ctx* ctx, int mask);
 /* Accumulated device time (ms) and launch count of one kernel kind. */
 int rtm_kernel_time(rtm_ctx* ctx, int kind, double* total_ms, long long* launches);
 

@@ -1,0 +1,284 @@
+// rtm_driver.cpp -- the reference's main() flow (kernel.cu:525-1213) on top of
+// librtm_hip.so, with a pthread shot dispatcher.
+//
+// Flow: read the two parameter files and the source depths; pad the model;
+// snap the velocity range and design the coefficient table; then shots are
+// grouped into device batches and handed to worker threads.  Each worker owns
+// one librtm_hip context (its own HIP stream and device buffers) and runs
+// forward pass, backward pass with imaging and the per-shot image post for
+// its batch, writing RVSP_RTM_up_<m>.dat / RVSP_RTM_down_<m>.dat exactly as
+// kernel.cu:951-990 does.  Those files are the checkpoints: a rerun skips
+// shots whose two files exist.  A failed batch is re-queued up to `retries`
+// times.  Finally the per-shot images are stacked in shot order
+// (kernel.cu:1000-1059), cropped (kernel.cu:1061-1108) and written as SEG-Y
+// (kernel.cu:1181-1209).  The depth-to-time / phase-rotation / time-to-depth
+// side files of kernel.cu:1110-1179 are not produced.
+#include <pthread.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../../include/rtm_hip.h"
+#include "../../../include/rtm_host.h"
+#include "rtm_host_internal.h"
+
+namespace {
+
+using rtmh::set_error;
+
+struct Batch {
+  std::vector<int> shots;  // 0-based source indices m
+  int attempts = 0;
+};
+
+struct Job {
+  rtmh_config cfg;
+  rtmh_derived d;
+  rtmh_run_opts opt;
+  std::vector<float> v, r1, w, c, wav, depths;
+  std::vector<int> index;
+  int nvel = 0;
+  float vmin = 0, vmax = 0;
+  std::mutex mu;
+  std::deque<Batch> queue;
+  int failures = 0;
+  std::string first_error;
+  int done_shots = 0;
+};
+
+bool file_exists(const std::string& p) {
+  struct stat st;
+  return stat(p.c_str(), &st) == 0;
+}
+
+std::string shot_file(const Job& J, const char* kind, int m) {
+  char name[64];
+  snprintf(name, sizeof(name), "RVSP_RTM_%s_%d.dat", kind, m + 1);
+  return std::string(J.cfg.result) + name;
+}
+
+// Gather file of kernel.cu:827-838: <OutNameseis>NEW_L10-1932-X_<N>.dat with
+// N the integer depth; n traces of NT1 floats.
+int read_gather(const Job& J, int m, float* out) {
+  char name[64];
+  snprintf(name, sizeof(name), "NEW_L10-1932-X_%d.dat", (int)J.depths[m]);
+  std::string p = std::string(J.cfg.out_seis) + name;
+  FILE* f = fopen(p.c_str(), "rb");
+  if (!f) return set_error(rtmh::RTMH_E_IO, "cannot open gather %s", p.c_str());
+  const size_t n = (size_t)J.cfg.n * J.cfg.nt1;
+  const size_t got = fread(out, sizeof(float), n, f);
+  fclose(f);
+  if (got != n) return set_error(rtmh::RTMH_E_FORMAT, "%s: %zu of %zu floats", p.c_str(), got, n);
+  return rtmh::RTMH_OK;
+}
+
+int write_interior_colmajor(const std::string& path, const float* img, int mnz, int mnx) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return set_error(rtmh::RTMH_E_IO, "cannot create %s", path.c_str());
+  std::vector<float> col(mnz);
+  for (int x = 0; x < mnx; ++x) {  // j outer, i inner (kernel.cu:954-959)
+    for (int z = 0; z < mnz; ++z) col[z] = img[(size_t)z * mnx + x];
+    fwrite(col.data(), sizeof(float), mnz, f);
+  }
+  fclose(f);
+  return rtmh::RTMH_OK;
+}
+
+int run_batch(Job& J, rtm_ctx* ctx, const Batch& b) {
+  const rtmh_config& c = J.cfg;
+  const int nb = (int)b.shots.size();
+  const int mnz = c.mod_nz, mnx = c.mod_nx;
+  std::vector<int> ru(nb), rx(nb, J.d.r_x);
+  std::vector<float> seis((size_t)nb * c.n * J.d.nt);
+  for (int i = 0; i < nb; ++i) {
+    const int m = b.shots[i];
+    ru[i] = rtmh_source_row(J.depths[m], c.hz, c.n2);
+    int rc = read_gather(J, m, &seis[(size_t)i * c.n * J.d.nt]);
+    if (rc) return rc;
+  }
+  int rc = rtm_set_shots(ctx, nb, ru.data(), rx.data());
+  if (!rc) rc = rtm_set_gathers(ctx, seis.data());
+  if (!rc) rc = rtm_migrate(ctx);
+  std::vector<float> up((size_t)nb * mnz * mnx), down((size_t)nb * mnz * mnx);
+  if (!rc) rc = rtm_image_post(ctx, J.vmax, c.whitecoe, up.data(), down.data());
+  if (rc) return set_error(rtmh::RTMH_E_DEVICE, "device batch failed (%d): %s", rc, rtm_last_error(ctx));
+  for (int i = 0; i < nb; ++i) {
+    const int m = b.shots[i];
+    rc = write_interior_colmajor(shot_file(J, "up", m), &up[(size_t)i * mnz * mnx], mnz, mnx);
+    if (!rc) rc = write_interior_colmajor(shot_file(J, "down", m), &down[(size_t)i * mnz * mnx], mnz, mnx);
+    if (rc) return rc;
+  }
+  return rtmh::RTMH_OK;
+}
+
+void* worker(void* arg) {
+  Job& J = *static_cast<Job*>(arg);
+  const rtmh_config& c = J.cfg;
+  rtm_params p{};
+  p.nz = J.d.nz; p.nx = J.d.nx; p.n2 = c.n2; p.nfdmax = c.nfdmax; p.nt = J.d.nt; p.nt2 = J.d.nt2;
+  p.ilste = c.ilste; p.icompen = c.icompen;
+  p.s_l = J.d.s_l; p.s_r = J.d.s_r; p.s_z = J.d.s_z; p.ds = c.ds; p.n = c.n;
+  p.tao2 = J.d.tao2; p.h2 = J.d.h2; p.taoh = J.d.taoh; p.taoh2 = J.d.taoh2; p.hzx2_1 = J.d.hzx2_1;
+  p.max_shots = J.opt.batch; p.strip_format = J.opt.strip_format; p.record = 0; p.zseg = 0;
+  rtm_ctx* ctx = nullptr;
+  int rc = rtm_create(&p, &ctx);
+  if (!rc)
+    rc = rtm_set_model(ctx, J.v.data(), J.r1.data(), J.w.data(), J.c.data(), (int)J.c.size(),
+                       c.ilste == 0 ? J.index.data() : nullptr, J.nvel, J.vmin, c.dv);
+  if (!rc) rc = rtm_set_wavelet(ctx, J.wav.data());
+  if (rc) {
+    std::lock_guard<std::mutex> g(J.mu);
+    J.failures++;
+    if (J.first_error.empty())
+      J.first_error = std::string("context setup failed: ") + (ctx ? rtm_last_error(ctx) : "rtm_create");
+    if (ctx) rtm_destroy(ctx);
+    return nullptr;
+  }
+  for (;;) {
+    Batch b;
+    {
+      std::lock_guard<std::mutex> g(J.mu);
+      if (J.queue.empty()) break;
+      b = J.queue.front();
+      J.queue.pop_front();
+    }
+    rc = run_batch(J, ctx, b);
+    std::lock_guard<std::mutex> g(J.mu);
+    if (rc) {
+      if (++b.attempts <= J.opt.retries) {
+        J.queue.push_back(b);
+      } else {
+        J.failures++;
+        if (J.first_error.empty()) J.first_error = rtmh_last_error();
+      }
+    } else {
+      J.done_shots += (int)b.shots.size();
+      if (J.opt.verbose) fprintf(stderr, "rtm_driver: %d shots done\n", J.done_shots);
+    }
+  }
+  rtm_destroy(ctx);
+  return nullptr;
+}
+
+}  // namespace
+
+extern "C" int rtmh_run(const char* rtm_txt, const rtmh_run_opts* opts) {
+  Job J;
+  J.opt = opts ? *opts : rtmh_run_opts{1, 8, 1, 0, 1, 0, nullptr};
+  if (J.opt.workers < 1) J.opt.workers = 1;
+  if (J.opt.batch < 1) J.opt.batch = 1;
+  int rc = rtmh_read_config(rtm_txt, &J.cfg);
+  if (rc) return rc;
+  rtmh_config& c = J.cfg;
+  rtmh_derive(&c, &J.d);
+  if (J.d.nt != c.nt1)
+    return set_error(rtmh::RTMH_E_ARG, "NT=%d differs from NT1=%d: trace resampling is not supported", J.d.nt, c.nt1);
+  J.depths.resize(c.nrec);
+  if ((rc = rtmh_read_depths(c.out_dpr, c.nrec, J.depths.data()))) return rc;
+  std::vector<float> raw((size_t)c.mod_nz * c.mod_nx);
+  if ((rc = rtmh_read_model(c.out_vp, c.mod_nz, c.mod_nx, raw.data()))) return rc;
+  const size_t nxz = (size_t)J.d.nz * J.d.nx;
+  J.v.resize(nxz);
+  J.r1.resize(nxz);
+  rtmh_pad_model(raw.data(), c.mod_nz, c.mod_nx, c.n2, c.ifv, c.tao, c.h, J.v.data(), J.r1.data());
+  rtmh_vrange(J.v.data(), (long long)nxz, c.dv, &J.vmin, &J.vmax, &J.nvel);
+  J.w.resize(c.n2 + 1);
+  rtmh_blend_weights(c.n2, J.w.data());
+  J.wav.resize(J.d.nt + 1);
+  rtmh_wavelet_table(J.d.nt, c.tao, c.f0, J.wav.data());
+  if (c.ilste == 0) {
+    std::vector<int> need(J.nvel), M(J.nvel);
+    J.index.resize(J.nvel + 1);
+    rtmh_bins_needed(J.v.data(), (long long)nxz, J.vmin, c.dv, J.nvel, need.data());
+    J.c.resize((size_t)J.nvel * (c.nfdmax + 1));
+    int nc = rtmh_design_ls(c.nthita, c.nfdmax, c.nfdmin, J.nvel, c.tao, c.h, c.df, c.eps, c.fmax,
+                            J.vmin, c.dv, need.data(), c.hz / c.h, M.data(), J.index.data(),
+                            J.c.data(), (int)J.c.size(), 16);
+    if (nc < 0) return nc;
+    J.c.resize(nc);
+  } else {
+    J.c.resize(c.nfdmax + 1);
+    rtmh_taylor(c.nfdmax, J.c.data());
+  }
+  // batches of shots still to do
+  Batch cur;
+  for (int m = 0; m < c.nrec; ++m) {
+    if (J.opt.resume && file_exists(shot_file(J, "up", m)) && file_exists(shot_file(J, "down", m))) continue;
+    cur.shots.push_back(m);
+    if ((int)cur.shots.size() == J.opt.batch) {
+      J.queue.push_back(cur);
+      cur.shots.clear();
+    }
+  }
+  if (!cur.shots.empty()) J.queue.push_back(cur);
+  const int nw = std::min<int>(J.opt.workers, std::max<size_t>(J.queue.size(), 1));
+  std::vector<pthread_t> th(nw);
+  for (int i = 0; i < nw; ++i) pthread_create(&th[i], nullptr, worker, &J);
+  for (int i = 0; i < nw; ++i) pthread_join(th[i], nullptr);
+  if (J.failures) return set_error(rtmh::RTMH_E_DEVICE, "%d batch(es) failed: %s", J.failures, J.first_error.c_str());
+
+  // stack the per-shot files in shot order (kernel.cu:1000-1059)
+  const int mnz = c.mod_nz, mnx = c.mod_nx;
+  const size_t nI = (size_t)mnz * mnx;
+  std::vector<float> s1(nI, 0.f), s2(nI, 0.f), col(nI);
+  for (int m = 0; m < c.nrec; ++m) {
+    for (int k = 0; k < 2; ++k) {
+      std::string p = shot_file(J, k ? "down" : "up", m);
+      FILE* f = fopen(p.c_str(), "rb");
+      if (!f || fread(col.data(), sizeof(float), nI, f) != nI) {
+        if (f) fclose(f);
+        return set_error(rtmh::RTMH_E_IO, "cannot read %s", p.c_str());
+      }
+      fclose(f);
+      std::vector<float>& s = k ? s2 : s1;
+      for (int x = 0; x < mnx; ++x)
+        for (int z = 0; z < mnz; ++z) s[(size_t)z * mnx + x] += col[(size_t)x * mnz + z];
+    }
+  }
+  std::vector<float> img(nI);
+  for (size_t i = 0; i < nI; ++i) {
+    float a = s1[i] / c.nrec, b = s2[i] / c.nrec;
+    img[i] = (c.inorm == 1) ? a / b : a;
+  }
+  // crop and flip (kernel.cu:1061-1108); the image is in interior coordinates
+  const std::string res(c.result);
+  std::vector<float> crop;
+  FILE* f = fopen((res + "RVSP_Migration_Real_new2.dat").c_str(), "wb");
+  if (!f) return set_error(rtmh::RTMH_E_IO, "cannot create output in %s", c.result);
+  auto put_img = [&](int z, int x) {
+    float val = (z >= 0 && z < mnz && x >= 0 && x < mnx) ? img[(size_t)z * mnx + x] : 0.f;
+    crop.push_back(val);
+  };
+  if (c.ifv == 1) {
+    for (int j = c.nx_ed; j > c.nx_bg; --j)
+      for (int i = c.nz_bg; i < c.nz_ed; ++i) put_img(i, j);
+  } else {
+    for (int j = c.nx_bg; j < c.nx_ed; ++j)
+      for (int i = c.nz_bg; i < c.nz_ed; ++i) put_img(i, j);
+  }
+  fwrite(crop.data(), sizeof(float), crop.size(), f);
+  fclose(f);
+  // SEG-Y output (kernel.cu:1181-1209): SGY_N traces of mod_NZ samples read
+  // sequentially from the cropped image.
+  const int sgy_n = c.nx_ed - c.nx_bg;
+  if (sgy_n > 0) {
+    std::vector<float> data((size_t)sgy_n * mnz, 0.f), sx(sgy_n), sy(sgy_n), dsr(sgy_n);
+    std::copy_n(crop.begin(), std::min(crop.size(), data.size()), data.begin());
+    for (int i = 0; i < sgy_n; ++i) {
+      sx[i] = (float)(1.0 * i * 10);
+      sy[i] = (float)(-1.0 * i * 10);
+      dsr[i] = (float)(1000 - i);
+    }
+    const char* tpl = J.opt.sgy_template ? J.opt.sgy_template : "SGY_Model.sgy";
+    rc = rtmh_write_segy(tpl, (res + "RVSP_migration_Real.sgy").c_str(), data.data(), sgy_n, mnz,
+                         (int)c.hz, sx.data(), sy.data(), 1.0f, 1.0f, dsr.data());
+    if (rc) return rc;
+  }
+  return rtmh::RTMH_OK;
+}

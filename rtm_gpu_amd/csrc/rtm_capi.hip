@@ -34,7 +34,7 @@ struct rtm_ctx {
   Model m{};
   Rcv r{};
   Strips st{};
-  int nfd = 0;           // compile-time stencil half-length used
+  int nfd = 0;           // compile-time stencil half-length: longest operator of the table
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -42,6 +42,7 @@ struct rtm_ctx {
   std::vector<void*> allocs;
   // device buffers
   float *d_v = nullptr, *d_r1 = nullptr, *d_ctab = nullptr, *d_w = nullptr;
+  size_t ctab_n = 0;
   float* F[3] = {nullptr, nullptr, nullptr};
   float* Q[3] = {nullptr, nullptr, nullptr};
   float *SU = nullptr, *SQ = nullptr, *I1 = nullptr, *I2 = nullptr;
@@ -60,7 +61,7 @@ struct rtm_ctx {
   bool model_ok = false, wav_ok = false, shots_ok = false, data_ok = false, save_strips = true;
   int zs = 32, nxt = 1, nblk = 1;
   // timing
-  bool timing = false;
+  int timing = 0;  // bit set of kernel kinds
   std::vector<EventPair> pending;
   std::vector<hipEvent_t> pool;
   double t_ms[RTM_K_NKINDS] = {0};
@@ -143,15 +144,16 @@ struct Timed {
   rtm_ctx* c;
   int kind;
   hipEvent_t a = nullptr, b = nullptr;
-  Timed(rtm_ctx* c_, int k) : c(c_), kind(k) {
-    if (c->timing) {
+  bool on;
+  Timed(rtm_ctx* c_, int k) : c(c_), kind(k), on((c_->timing >> k) & 1) {
+    if (on) {
       a = get_event(c);
       b = get_event(c);
       if (a && b) (void)hipEventRecord(a, c->stream);
     }
   }
   ~Timed() {
-    if (c->timing && a && b) {
+    if (on && a && b) {
       (void)hipEventRecord(b, c->stream);
       c->pending.push_back({a, b, kind});
       if (c->pending.size() >= 4096) drain_timing(c);
@@ -400,13 +402,21 @@ int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, c
                   int nc, const int* index, int nvel, float vmin, float dv) {
   if (!c || !v || !r1 || !w || !coef) return RTM_E_ARG;
   const Grid& g = c->g;
-  const int nfd = c->nfd;
-  const int CS = ((2 + nfd) + 3) & ~3;
   const bool ls = (c->p.ilste == 0);
   if (ls && (!index || nvel < 1)) return fail(c, RTM_E_ARG, "LS model needs index[nvel+1]");
   if (!ls && nc < c->p.nfdmax + 1) return fail(c, RTM_E_ARG, "Taylor table needs nfdmax+1 coefficients");
   if (!(dv > 0.f)) return fail(c, RTM_E_ARG, "dv must be > 0");
   const int nrow = ls ? nvel : 1;
+  // The kernels run the longest operator of the table: every shorter row is
+  // zero-padded, and w1 + 0*s == w1, so the taps beyond a bin's M change
+  // nothing.  The strips keep their nfdmax rings (reference layout).
+  int nfd = c->p.nfdmax;
+  if (ls) {
+    nfd = 1;
+    for (int k = 0; k < nvel; ++k) nfd = std::max(nfd, index[k + 1] - index[k] - 1);
+  }
+  if (nfd > c->p.nfdmax) return fail(c, RTM_E_ARG, "table operator length %d > nfdmax %d", nfd, c->p.nfdmax);
+  const int CS = ((2 + nfd) + 3) & ~3;
   std::vector<float> tab((size_t)nrow * CS, 0.f);
   const double h1 = 1.0 + (double)c->p.hzx2_1;
   for (int k = 0; k < nrow; ++k) {
@@ -420,9 +430,10 @@ int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, c
     for (int l = 1; l <= M; ++l) tab[(size_t)k * CS + 1 + l] = coef[top + l];
   }
   int rc = RTM_OK;
-  if (!c->d_ctab || c->m.nrow != nrow) {
+  if (!c->d_ctab || c->ctab_n < tab.size()) {
     rc = dalloc_t(c, &c->d_ctab, tab.size());
     if (rc != RTM_OK) return rc;
+    c->ctab_n = tab.size();
   }
   HIPCHK(c, hipMemcpyAsync(c->d_ctab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
   // model arrays into the padded layout; guard columns repeat the edge value
@@ -450,6 +461,7 @@ int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, c
   m.inv_dv = m.dv_pow2 ? ldexpf(1.0f, 1 - ex) : 1.0f / dv;
   m.hzx2_1 = c->p.hzx2_1; m.tao2 = c->p.tao2; m.h2 = c->p.h2;
   m.taoh = c->p.taoh; m.taoh2 = c->p.taoh2;
+  c->nfd = nfd;
   c->model_ok = true;
   return RTM_OK;
 }
@@ -493,6 +505,17 @@ int rtm_set_gathers(rtm_ctx* c, const float* seis) {
     HIPCHK(c, hipMemcpy(c->d_data + (size_t)s * nt * n, t.data(), t.size() * sizeof(float),
                         hipMemcpyHostToDevice));
   }
+  c->data_ok = true;
+  return RTM_OK;
+}
+
+int rtm_gathers_from_records(rtm_ctx* c) {
+  if (!c) return RTM_E_ARG;
+  if (!c->p.record || !c->d_rec) return fail(c, RTM_E_STATE, "context created without records");
+  if (c->kf != c->p.nt) return fail(c, RTM_E_STATE, "no complete forward pass recorded");
+  // records and gathers share the time-major [shot][nt][n] layout
+  HIPCHK(c, hipMemcpyAsync(c->d_data, c->d_rec, sizeof(float) * c->nshot * c->p.nt * c->p.n,
+                           hipMemcpyDeviceToDevice, c->stream));
   c->data_ok = true;
   return RTM_OK;
 }
@@ -672,11 +695,11 @@ int rtm_stack_finish(rtm_ctx* c, int nrec, int inorm, float* img, float* illum) 
   return RTM_OK;
 }
 
-int rtm_set_timing(rtm_ctx* c, int on) {
+int rtm_set_timing(rtm_ctx* c, int mask) {
   if (!c) return RTM_E_ARG;
   int rc = drain_timing(c);
   if (rc != RTM_OK) return rc;
-  c->timing = on != 0;
+  c->timing = mask;
   for (int i = 0; i < RTM_K_NKINDS; ++i) { c->t_ms[i] = 0; c->t_n[i] = 0; }
   return RTM_OK;
 }

@@ -55,6 +55,7 @@ def lib():
         L.rtm_set_wavelet.argtypes = [vp, _fp]
         L.rtm_set_shots.argtypes = [vp, C.c_int, _ip, _ip]
         L.rtm_set_gathers.argtypes = [vp, _fp]
+        L.rtm_gathers_from_records.argtypes = [vp]
         L.rtm_forward_begin.argtypes = [vp, C.c_int]
         L.rtm_forward_advance.argtypes = [vp, C.c_int]
         L.rtm_backward_begin.argtypes = [vp]
@@ -69,7 +70,7 @@ def lib():
         L.rtm_set_timing.argtypes = [vp, C.c_int]
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
-                  "rtm_set_gathers", "rtm_forward_begin", "rtm_forward_advance",
+                  "rtm_set_gathers", "rtm_gathers_from_records", "rtm_forward_begin", "rtm_forward_advance",
                   "rtm_backward_begin", "rtm_backward_advance", "rtm_migrate", "rtm_sync",
                   "rtm_read_field", "rtm_read_records", "rtm_read_strip", "rtm_image_post",
                   "rtm_stack_finish", "rtm_set_timing", "rtm_kernel_time"):
@@ -132,6 +133,9 @@ class Context:
         self._chk(lib().rtm_set_gathers(self.h, np.ascontiguousarray(seis, np.float32).ravel()),
                   "rtm_set_gathers")
 
+    def gathers_from_records(self):
+        self._chk(lib().rtm_gathers_from_records(self.h), "rtm_gathers_from_records")
+
     def forward_begin(self, save_strips=True):
         self._chk(lib().rtm_forward_begin(self.h, int(bool(save_strips))), "rtm_forward_begin")
 
@@ -183,8 +187,9 @@ class Context:
                   "rtm_stack_finish")
         return img.reshape(mnz, mnx), ill.reshape(mnz, mnx)
 
-    def set_timing(self, on):
-        self._chk(lib().rtm_set_timing(self.h, int(bool(on))), "rtm_set_timing")
+    def set_timing(self, mask):
+        """mask: bit set of kernel kinds (1 << K_*), 0 off, -1 all."""
+        self._chk(lib().rtm_set_timing(self.h, int(mask)), "rtm_set_timing")
 
     def kernel_time(self, kind):
         ms = C.c_double()

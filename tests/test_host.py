@@ -1,0 +1,232 @@
+"""The product host library (librtm_host.so) against the oracle, the compiled
+reference helpers and the SEG-Y golden fixture.  CPU only."""
+import os
+import shutil
+import tempfile
+
+import numpy as np
+import pytest
+
+import cases
+import oracle_bind as ob
+from rtm_gpu_amd import host
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("args", [
+    (200, 400, 20, 2000, 0.001, 0.001, 10.0, 10.0, 25.0, 1, 1, 400, 1, 201),
+    (351, 1701, 30, 4000, 0.001, 0.001, 12.5, 12.5, 20.0, 1, 1, 1701, 1, 851),
+    (801, 4001, 30, 5000, 0.0008, 0.0008, 10.0, 10.0, 15.0, 1, 1, 4001, 1, 2001),
+    (256, 1024, 20, 3000, 0.001, 0.0005, 10.0, 20.0, 25.0, 3, 2, 300, 3, 500)])
+def test_derive_matches_oracle(args):
+    mod_nz, mod_nx, n2, nt1, tao1, tao, h, hz, f0, s_l1, s_z1, n, ds, r_x1 = args
+    o = ob.derive(*args)
+    c = host.make_config(mod_nz=mod_nz, mod_nx=mod_nx, n2=n2, nt1=nt1, tao1=tao1, tao=tao, h=h,
+                         hz=hz, f0=f0, s_l1=s_l1, s_z1=s_z1, n=n, ds=ds, r_x1=r_x1)
+    d = host.derive(c)
+    for k, dk in (("NZ", "nz"), ("NX", "nx"), ("NT", "nt"), ("NT2", "nt2"), ("s_l", "s_l"),
+                  ("s_r", "s_r"), ("s_z", "s_z"), ("r_x", "r_x")):
+        assert o[k] == getattr(d, dk), k
+    for k in ("taoh", "tao2", "h2", "taoh2", "hzx", "hzx2_1"):
+        assert np.float32(getattr(d, k)) == o[k], k
+
+
+def test_source_rows_and_wavelet_match_oracle():
+    for depth in np.arange(0.0, 1000.0, 6.25):
+        for hz, n2 in ((12.5, 30), (10.0, 20), (7.0, 9)):
+            assert host.source_row(depth, hz, n2) == ob.lib().orc_source_row(
+                np.float32(depth), np.float32(hz), n2)
+    for nt, tao, f0 in ((400, 0.001, 25.0), (300, 0.0008, 15.0), (200, 0.0005, 20.0)):
+        a = host.wavelet_table(nt, tao, f0)
+        b = ob.wavelet_table(nt, tao, f0)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for n2 in (1, 7, 20, 30):
+        assert np.array_equal(host.blend_weights(n2), ob.blend_weights(n2))
+
+
+@pytest.mark.parametrize("ifv", [0, 1])
+def test_pad_model_and_bins_match_oracle(ifv):
+    raw = cases.layered_model(33, 47, 5, 1500, 5500, seed=11, dip=0.4)
+    raw = raw + np.random.default_rng(3).uniform(-30, 30, raw.shape).astype(np.float32)
+    v, r1 = host.pad_model(raw, 33, 47, 9, ifv, 0.001, 12.5)
+    vo, r1o = ob.pad_model(raw, 33, 47, 9, ifv, 0.001, 12.5)
+    assert np.array_equal(v, vo) and np.array_equal(r1.view(np.uint32), r1o.view(np.uint32))
+    for dv in (1.0, 2.0, 0.5, 3.0):
+        a = host.vrange(v, dv)
+        b = ob.vrange(v, np.float32(dv))
+        assert a == b
+        assert np.array_equal(host.bins_needed(v, a[0], dv, a[2]),
+                              ob.bins_needed(v, a[0], np.float32(dv), a[2]))
+
+
+@pytest.mark.parametrize("vmin,vmax,nfdmax,h,tao,fmax,hzx,every", [
+    (1500.0, 4500.0, 4, 10.0, 0.001, 51.0, 1.0, 37),
+    (1500.0, 5500.0, 10, 12.5, 0.001, 41.0, 1.0, 101),
+    (1800.0, 2600.0, 6, 10.0, 0.0008, 31.0, 2.0, 7)])
+def test_design_ls_matches_oracle_and_reference(vmin, vmax, nfdmax, h, tao, fmax, hzx, every):
+    nvel = int((vmax - vmin) / 1.0 + 1.5)
+    need = np.zeros(nvel, np.int32)
+    need[::every] = 1
+    need[-1] = 1
+    a = host.design_ls(1000, nfdmax, 2, nvel, tao, h, 1.0, 1e-5, fmax, vmin, 1.0, need, hzx, 8)
+    a1 = host.design_ls(1000, nfdmax, 2, nvel, tao, h, 1.0, 1e-5, fmax, vmin, 1.0, need, hzx, 1)
+    o = ob.design_ls(1000, nfdmax, 2, nvel, tao, h, 1.0, 1e-5, fmax, vmin, 1.0, need, hzx)
+    for x, y in zip(a, o):
+        assert np.array_equal(x.view(np.int32), y.view(np.int32))
+    for x, y in zip(a, a1):
+        assert np.array_equal(x.view(np.int32), y.view(np.int32))
+    if ob.ref_available():
+        r = ob.ref_design_ls(1000, nfdmax, 2, nvel, tao, h, 1.0, 1e-5, fmax, vmin, vmax, 1.0, need,
+                             hzx)
+        for x, y in zip(a, r):
+            assert np.array_equal(x.view(np.int32), y.view(np.int32))
+
+
+@pytest.mark.parametrize("M", [1, 3, 4, 8, 10, 12])
+def test_taylor_matches_oracle(M):
+    assert np.array_equal(host.taylor(M).view(np.uint32), ob.taylor(M).view(np.uint32))
+
+
+def test_stack_matches_oracle():
+    rng = np.random.default_rng(5)
+    ups = rng.normal(0, 1, (7, 13, 17)).astype(np.float32)
+    downs = rng.uniform(0.5, 2, (7, 13, 17)).astype(np.float32)
+    for inorm in (0, 1):
+        a = host.stack(ups, downs, inorm)
+        b = ob.stack(ups, downs, inorm)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+# ---- SEG-Y -------------------------------------------------------------------
+def test_ibm_known_answers():
+    b = host.float_to_ibm(np.array([1.0, -118.625, 0.0, 0.15625], np.float32))
+    words = [int.from_bytes(bytes(b[4 * i:4 * i + 4]), "big") for i in range(4)]
+    assert words[:3] == [0x41100000, 0xC276A000, 0x00000000]
+    assert words[3] == 0x40280000  # 0.15625 = 0x0.28 * 16^0
+    x = np.array([1.0, -118.625, 0.5, 3.0, -0.15625, 1024.0], np.float32)
+    assert np.array_equal(host.ibm_to_float(host.float_to_ibm(x)), x)
+
+
+def test_ibm_matches_reference_codec():
+    if not ob.ref_available():
+        pytest.skip("oracle/_ref not built")
+    rng = np.random.default_rng(9)
+    x = (rng.normal(0, 1, 4096) * np.logspace(-35, 35, 4096)).astype(np.float32)
+    x[:8] = [0.0, -0.0, 1.0, -1.0, 3.4e38, -1.2e-38, 1e-45, 5.877472e-39]
+    buf = C_buf = bytes(4 * x.size)
+    import ctypes
+    C_buf = ctypes.create_string_buffer(4 * x.size)
+    ob.ref().ref_trace2segy(C_buf, x, x.size, 1)
+    ref_bytes = np.frombuffer(C_buf.raw, np.uint8)
+    assert np.array_equal(host.float_to_ibm(x), ref_bytes)
+    back = np.zeros(x.size, np.float32)
+    ob.ref().ref_segy2trace(C_buf.raw, back, x.size, 1)
+    assert np.array_equal(host.ibm_to_float(ref_bytes).view(np.uint32), back.view(np.uint32))
+    del buf
+
+
+def test_segy_writer_matches_golden():
+    img = np.load(os.path.join(GOLDEN, "segy_input.npy"))
+    ntr = img.shape[0]
+    sx = np.array([1.0 * i * 10 for i in range(ntr)], np.float32)
+    sy = np.array([-1.0 * i * 10 for i in range(ntr)], np.float32)
+    dsr = np.array([1000 - i for i in range(ntr)], np.float32)
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "out.sgy")
+        host.write_segy(os.path.join(GOLDEN, "SGY_Model.sgy"), out, img, 10, sx, sy, 1.0, 1.0, dsr)
+        got = open(out, "rb").read()
+        data, fmt = host.read_segy(out)
+    exp = open(os.path.join(GOLDEN, "segy_expected.sgy"), "rb").read()
+    assert got == exp
+    assert fmt == 1 and data.shape == img.shape
+    # IBM round trip loses at most the 3 low fraction bits of base-16
+    # alignment; results below the IEEE normal range decode to zero
+    # (segy.cpp:603-651 sets the exponent only when it is positive).
+    nz = np.abs(img) >= np.finfo(np.float32).tiny
+    assert np.all(np.abs(data[nz] - img[nz]) <= 2.0 ** -20 * np.abs(img[nz]))
+    assert np.all(data[~nz] == 0)
+
+
+def test_segy_writer_matches_reference_live(tmp_path):
+    if not ob.ref_available():
+        pytest.skip("oracle/_ref not built")
+    rng = np.random.default_rng(17)
+    img = rng.normal(0, 3, (25, 61)).astype(np.float32)
+    sx = rng.uniform(0, 1e4, 25).astype(np.float32)
+    sy = rng.uniform(-1e4, 0, 25).astype(np.float32)
+    dsr = rng.uniform(-500, 500, 25).astype(np.float32)
+    shutil.copy(os.path.join(GOLDEN, "SGY_Model.sgy"), tmp_path / "SGY_Model.sgy")
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        ob.ref().ref_write_sgy(img.ravel(), 25, 61, 12, sx, sy, 3.7, -2.2, dsr, b"ref.sgy")
+    finally:
+        os.chdir(cwd)
+    host.write_segy(tmp_path / "SGY_Model.sgy", tmp_path / "ours.sgy", img, 12, sx, sy, 3.7, -2.2,
+                    dsr)
+    assert (tmp_path / "ours.sgy").read_bytes() == (tmp_path / "ref.sgy").read_bytes()
+
+
+# ---- parameter files ---------------------------------------------------------
+def write_param_files(d, **over):
+    vals = dict(nfdmax=4, nfdmin=2, n2=20, f0=25.0, fmax=51.0, df=1.0, nthita=1000, eps=1e-5,
+                dv=1, ilste=0, ifv=0, whitecoe=1e-4, hz=10.0, tao=0.001, inorm=1, icompen=1,
+                nsmooth=0, wthite_phase=1e-3, angle=90, nx_bg=0, nx_ed=40, nz_bg=0, nz_ed=30,
+                h=10.0, tao1=0.001, mod_nz=30, mod_nx=40, nt1=300, s_l1=1, s_z1=1, n=40, ds=1,
+                r_x1=20, nrec=2, dr=1)
+    vals.update(over)
+    d = str(d)
+    labels = ["The Maximum Length Of Operator", "The Minimum Length Of Operator",
+              "The Hyrid Absorbing Boundary Width", "Dominant Frequency",
+              "Frequency parameter for optimal operator", "Interval Of Frequency",
+              "Azimuth Of The Plane Wave Divide Into The Number", "Dispersion Value",
+              "Velocity Interval", "LSM-0,TEM-1", "v-1 to reverse velocity", "white coefficient",
+              "Intervel of Space in Z", "Intervel of Time", "Normalization-1,NoNormlization-0",
+              "Compensate-1,Noncompensate-0", "Number of Smooth",
+              "white  coefficient for phase correction", "angle of phase correction",
+              "Trace Begin", "Trace End", "Depth Begin", "Depth End",
+              "The loacation of Output seismic data", "The location of the Output Velocity data",
+              "Depth of Receiver parameter", "Output Parameter", "Final migration result"]
+    keys = ["nfdmax", "nfdmin", "n2", "f0", "fmax", "df", "nthita", "eps", "dv", "ilste", "ifv",
+            "whitecoe", "hz", "tao", "inorm", "icompen", "nsmooth", "wthite_phase", "angle",
+            "nx_bg", "nx_ed", "nz_bg", "nz_ed"]
+    paths = [d + "/", d + "/model.dat", d + "/Depth_Of_Receiver.txt", d + "/Parameter.txt", d + "/"]
+    with open(os.path.join(d, "2D_Real_RVSP_RTM.txt"), "w") as f:
+        for lab, k in zip(labels, keys):
+            f.write(f"{lab}\n{vals[k]}\n")
+        for lab, p in zip(labels[23:], paths):
+            f.write(f"{lab}\n{p}\n")
+    with open(os.path.join(d, "Parameter.txt"), "w") as f:
+        f.write(" \n".join(str(vals[k]) for k in ("h", "tao1", "mod_nz", "mod_nx", "nt1", "s_l1",
+                                                  "s_z1", "n", "ds", "r_x1", "nrec", "dr")))
+    return vals
+
+
+def test_read_config_reference_layout(tmp_path):
+    vals = write_param_files(tmp_path)
+    c = host.read_config(tmp_path / "2D_Real_RVSP_RTM.txt")
+    for k in ("nfdmax", "nfdmin", "n2", "nthita", "ilste", "ifv", "inorm", "icompen", "nx_ed",
+              "nz_ed", "mod_nz", "mod_nx", "nt1", "n", "r_x1", "nrec"):
+        assert getattr(c, k) == vals[k], k
+    for k in ("f0", "fmax", "eps", "hz", "tao", "whitecoe", "h", "tao1"):
+        assert getattr(c, k) == np.float32(vals[k]), k
+    assert c.out_vp.decode() == str(tmp_path) + "/model.dat"
+    assert c.result.decode() == str(tmp_path) + "/"
+
+
+def test_read_config_reference_file():
+    # the reference's own parameter file parses (its OutPara path is absent)
+    ref_txt = "/root/reference/2D_Real_RVSP_RTM.txt"
+    if not os.path.exists(ref_txt):
+        pytest.skip("reference tree absent")
+    with pytest.raises(host.HostError, match="OutPara"):
+        host.read_config(ref_txt)
+
+
+def test_missing_files_are_clean_errors(tmp_path):
+    with pytest.raises(host.HostError, match="cannot open"):
+        host.read_config(tmp_path / "nope.txt")
+    write_param_files(tmp_path)
+    with pytest.raises(host.HostError, match="cannot open depth file|cannot open model"):
+        host.run(tmp_path / "2D_Real_RVSP_RTM.txt")
