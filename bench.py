@@ -136,8 +136,23 @@ def cpu_baseline(P, target_s=15.0):
                       f"grid, {el:.1f} s"}
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size (read mib MiB, write mib
+    MiB), run under rocprofv3 --pmc to calibrate FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = mib * 2 ** 20 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run only a MIB-sized copy kernel (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
@@ -147,6 +162,9 @@ def main():
     ap.add_argument("--zseg", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {a.gpus} requested); shots shard across "
               "GPUs with one image all-reduce at the end, which this build does not run.",

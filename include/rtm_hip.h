@@ -110,6 +110,10 @@ int rtm_backward_begin(rtm_ctx* ctx);
 int rtm_backward_advance(rtm_ctx* ctx, int nsteps);
 /* Forward then backward for the whole batch. */
 int rtm_migrate(rtm_ctx* ctx);
+/* on = 1: rtm_migrate captures its launch sequence into a hipGraph on first
+ * use and replays it afterwards (until the model, wavelet or shots change).
+ * For small grids, where launch latency dominates. */
+int rtm_set_graph(rtm_ctx* ctx, int on);
 /* Block until the context's stream is idle. */
 int rtm_sync(rtm_ctx* ctx);
 

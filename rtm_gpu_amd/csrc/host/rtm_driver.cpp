@@ -264,6 +264,25 @@ extern "C" int rtmh_run(const char* rtm_txt, const rtmh_run_opts* opts) {
   }
   fwrite(crop.data(), sizeof(float), crop.size(), f);
   fclose(f);
+  // the velocity window beside it (vnew.dat, kernel.cu:1074-1107), padded
+  // coordinates; note the ifv = 0 loop starts at NZ_BG, not NZ_BG + N2
+  {
+    std::vector<float> vw;
+    auto put_v = [&](int z, int x) {
+      vw.push_back((z >= 0 && z < J.d.nz && x >= 0 && x < J.d.nx) ? J.v[(size_t)z * J.d.nx + x] : 0.f);
+    };
+    if (c.ifv == 1) {
+      for (int j = c.nx_ed + c.n2; j > c.nx_bg + c.n2; --j)
+        for (int i = c.nz_bg + c.n2; i < c.nz_ed + c.n2; ++i) put_v(i, j);
+    } else {
+      for (int j = c.nx_bg + c.n2; j < c.nx_ed + c.n2; ++j)
+        for (int i = c.nz_bg; i < c.nz_ed + c.n2; ++i) put_v(i, j);
+    }
+    FILE* fv = fopen((res + "vnew.dat").c_str(), "wb");
+    if (!fv) return set_error(rtmh::RTMH_E_IO, "cannot create vnew.dat in %s", c.result);
+    fwrite(vw.data(), sizeof(float), vw.size(), fv);
+    fclose(fv);
+  }
   // SEG-Y output (kernel.cu:1181-1209): SGY_N traces of mod_NZ samples read
   // sequentially from the cropped image.
   const int sgy_n = c.nx_ed - c.nx_bg;

@@ -60,6 +60,11 @@ struct rtm_ctx {
   int kf = -1, kb = -2;  // next forward / backward step; -1/-2 = not begun
   bool model_ok = false, wav_ok = false, shots_ok = false, data_ok = false, save_strips = true;
   int zs = 32, nxt = 1, nblk = 1;
+  // captured migration (hipGraph)
+  bool graph_on = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_f[3] = {0, 1, 2}, g_q[3] = {0, 1, 2}, g_kf = 0, g_kb = 0;
   // timing
   int timing = 0;  // bit set of kernel kinds
   std::vector<EventPair> pending;
@@ -283,6 +288,13 @@ int choose_zs(rtm_ctx* c, int nshot) {
   return RTM_OK;
 }
 
+void drop_graph(rtm_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -387,6 +399,7 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
 int rtm_destroy(rtm_ctx* c) {
   if (!c) return RTM_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   for (auto& ep : c->pending) { (void)hipEventDestroy(ep.a); (void)hipEventDestroy(ep.b); }
   for (auto e : c->pool) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
@@ -462,6 +475,7 @@ int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, c
   m.hzx2_1 = c->p.hzx2_1; m.tao2 = c->p.tao2; m.h2 = c->p.h2;
   m.taoh = c->p.taoh; m.taoh2 = c->p.taoh2;
   c->nfd = nfd;
+  drop_graph(c);
   c->model_ok = true;
   return RTM_OK;
 }
@@ -469,6 +483,7 @@ int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, c
 int rtm_set_wavelet(rtm_ctx* c, const float* wav) {
   if (!c || !wav) return RTM_E_ARG;
   c->wav.assign(wav, wav + c->p.nt + 1);
+  drop_graph(c);
   c->wav_ok = true;
   return RTM_OK;
 }
@@ -486,6 +501,7 @@ int rtm_set_shots(rtm_ctx* c, int nshot, const int* r_u, const int* r_x) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->nshot = nshot;
   choose_zs(c, nshot);
+  drop_graph(c);
   c->shots_ok = true;
   c->data_ok = false;
   c->kf = -1;
@@ -583,12 +599,59 @@ int rtm_backward_advance(rtm_ctx* c, int nsteps) {
   return RTM_OK;
 }
 
-int rtm_migrate(rtm_ctx* c) {
+static int migrate_eager(rtm_ctx* c) {
   int rc = rtm_forward_begin(c, 1);
   if (rc == RTM_OK) rc = rtm_forward_advance(c, c->p.nt);
   if (rc == RTM_OK) rc = rtm_backward_begin(c);
   if (rc == RTM_OK) rc = rtm_backward_advance(c, c->p.nt);
   return rc;
+}
+
+int rtm_migrate(rtm_ctx* c) {
+  if (!c) return RTM_E_ARG;
+  if (!c->graph_on) return migrate_eager(c);
+  if (!c->model_ok || !c->wav_ok || !c->shots_ok || !c->data_ok)
+    return fail(c, RTM_E_STATE, "model, wavelet, shots and gathers must be set");
+  if (!c->gexec) {
+    // Capture the whole launch sequence once; the host-side buffer rotation
+    // is deterministic, so the replay leaves the same state as the capture.
+    const int saved_timing = c->timing;
+    c->timing = 0;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = migrate_eager(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->timing = saved_timing;
+    if (rc != RTM_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, RTM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    c->graph = g;
+    e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graph(c);
+      return fail(c, RTM_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    c->g_f[0] = c->f0; c->g_f[1] = c->f1; c->g_f[2] = c->f2;
+    c->g_q[0] = c->q0; c->g_q[1] = c->q1; c->g_q[2] = c->q2;
+    c->g_kf = c->kf;
+    c->g_kb = c->kb;
+  }
+  HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  c->f0 = c->g_f[0]; c->f1 = c->g_f[1]; c->f2 = c->g_f[2];
+  c->q0 = c->g_q[0]; c->q1 = c->g_q[1]; c->q2 = c->g_q[2];
+  c->kf = c->g_kf;
+  c->kb = c->g_kb;
+  c->save_strips = true;
+  return RTM_OK;
+}
+
+int rtm_set_graph(rtm_ctx* c, int on) {
+  if (!c) return RTM_E_ARG;
+  drop_graph(c);
+  c->graph_on = on != 0;
+  return RTM_OK;
 }
 
 int rtm_sync(rtm_ctx* c) {

@@ -61,6 +61,7 @@ def lib():
         L.rtm_backward_begin.argtypes = [vp]
         L.rtm_backward_advance.argtypes = [vp, C.c_int]
         L.rtm_migrate.argtypes = [vp]
+        L.rtm_set_graph.argtypes = [vp, C.c_int]
         L.rtm_sync.argtypes = [vp]
         L.rtm_read_field.argtypes = [vp, C.c_int, C.c_int, _fp]
         L.rtm_read_records.argtypes = [vp, _fp]
@@ -71,7 +72,7 @@ def lib():
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
                   "rtm_set_gathers", "rtm_gathers_from_records", "rtm_forward_begin", "rtm_forward_advance",
-                  "rtm_backward_begin", "rtm_backward_advance", "rtm_migrate", "rtm_sync",
+                  "rtm_backward_begin", "rtm_backward_advance", "rtm_migrate", "rtm_set_graph", "rtm_sync",
                   "rtm_read_field", "rtm_read_records", "rtm_read_strip", "rtm_image_post",
                   "rtm_stack_finish", "rtm_set_timing", "rtm_kernel_time"):
             getattr(L, n).restype = C.c_int
@@ -150,6 +151,9 @@ class Context:
 
     def migrate(self):
         self._chk(lib().rtm_migrate(self.h), "rtm_migrate")
+
+    def set_graph(self, on):
+        self._chk(lib().rtm_set_graph(self.h, int(bool(on))), "rtm_set_graph")
 
     def sync(self):
         self._chk(lib().rtm_sync(self.h), "rtm_sync")

@@ -43,8 +43,12 @@ def build_case(name="small", mod_nz=40, mod_nx=60, n2=10, nfdmax=4, nfdmin=2, nt
     w = ob.blend_weights(n2)
     if ilste == 0:
         need = ob.bins_needed(v, vmin, dv, nvel)
-        M, index, c = ob.design_ls(1000, nfdmax, nfdmin, nvel, tao, h, 1.0, 1e-5, fmax,
-                                   float(vmin), float(dv), need, float(d["hzx"]))
+        # the reference holds tao, h, df, eps, fmax in float variables and
+        # promotes them to double for funMandC (kernel.cu:536, 746)
+        f32 = lambda x: float(np.float32(x))  # noqa: E731
+        M, index, c = ob.design_ls(1000, nfdmax, nfdmin, nvel, f32(tao), f32(h), f32(1.0),
+                                   f32(1e-5), f32(fmax), float(vmin), float(dv), need,
+                                   float(d["hzx"]))
     else:
         index = None
         c = ob.taylor(nfdmax)

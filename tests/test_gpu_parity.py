@@ -169,3 +169,26 @@ def test_image_post_and_stack():
         downs.append(d)
     oimg, oill = ob.stack(np.stack(ups), np.stack(downs), 1)
     assert np.array_equal(img, oimg) and np.array_equal(ill, oill)
+
+
+def test_graph_replay_matches_eager():
+    """hipGraph-captured migration: every replay equals the eager result,
+    with eager work and host synchronisation between the replays."""
+    from rtm_gpu_amd import hip
+    case = _case("frame_src")
+    seis = synth_gather(case)
+    eager = hip_context(case, nshot=1)
+    eager.set_gathers(seis[None])
+    eager.migrate()
+    ref_i1 = eager.field(hip.FIELD_I1)
+    ref_i2 = eager.field(hip.FIELD_I2)
+    g = hip_context(case, nshot=1)
+    g.set_gathers(seis[None])
+    g.set_graph(True)
+    for rep in range(4):
+        g.migrate()
+        g.sync()
+        assert np.array_equal(g.field(hip.FIELD_I1), ref_i1), f"replay {rep}"
+        assert np.array_equal(g.field(hip.FIELD_I2), ref_i2), f"replay {rep}"
+        eager.migrate()  # unrelated eager work between replays
+        eager.sync()
