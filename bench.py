@@ -251,7 +251,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(kd["gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kd["gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic(dom, workload),
+                     "traffic": pmc_traffic(dom, f"rvsp_faulted_351x1701_{a.shots}shots"),
                      "algorithmic_bytes_per_launch": kd["bytes_per_launch"],
                      "avg_launch_ms": round(kd["avg_ms"], 5)},
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "launches": v["launches"],
