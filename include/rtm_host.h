@@ -86,6 +86,24 @@ int rtmh_read_segy(const char* path, int* ntr, int* ns, int* format, float* data
 void rtmh_stack(const float* ups, const float* downs, int nshot, long long nI, int inorm,
                 float* img, float* illum);
 
+/* Post-stack output chain (kernel.cu:1110-1179).  Arrays are trace-major
+ * ([column][sample]); columns x_st..x_end of v are converted, d is indexed
+ * from its first trace.  Each returns the output length per trace (> 0) or a
+ * negative code; out (ntr*len floats, ntr = x_end-x_st+1) may be NULL to query
+ * the length, cap is its capacity in floats.
+ *   rtmh_d2t: depth -> two-way time, time step dt (D2T,
+ *             DisToTimeAndTimeToDis1D.cpp:115-173; called with dz = hz, dt = tao).
+ *   rtmh_t2d: time -> depth, time step dtt and depth step dzz over a velocity of
+ *             nzv samples per column (T2D, DisToTimeAndTimeToDis1D.cpp:31-112;
+ *             called with dtt = tao, dzz = hz). */
+int rtmh_d2t(const float* v, const float* d, int nx, int nz, int x_st, int x_end, float dz,
+             float dt, float* out, long long cap);
+int rtmh_t2d(const float* v, const float* d, int nx, int nt, int nzv, int x_st, int x_end,
+             float dtt, float dzz, float* out, long long cap);
+/* Constant-phase rotation of ntr traces of nt samples by angle degrees
+ * (phase_correction, phase_correction_ricker_decon.cpp:153-218). */
+void rtmh_phase_rotate(const float* din, float* dout, int ntr, int nt, float angle);
+
 /* The driver: runs the reference's main() flow from a 2D_Real_RVSP_RTM.txt
  * file.  workers = pthread workers, each with its own librtm_hip context and
  * stream; batch = shots per device batch.  Shots whose per-shot image files

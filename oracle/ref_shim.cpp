@@ -3,8 +3,10 @@
 // Thin extern "C" entry points over the reference's own pure-C++ helper
 // sources, compiled where they lie under the reference tree by
 // oracle/Makefile into oracle/_ref/libref.so.  Used only by tests to pin the
-// oracle (coefficient design, model padding, SEG-Y bytes).  Nothing here
+// oracle and the host library (coefficient design, model padding, SEG-Y bytes,
+// depth/time conversion and phase rotation).  Nothing here
 // restates or replaces reference code; each wrapper forwards to it.
+#include <malloc.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -21,6 +23,11 @@ void callenfd2d_ls(double* hk, double* disperr, int nfre, int nfdmin, int nfdmax
                    int iLSTE, double hzx);                          // LSMOrCon_rec_2D.cpp:293
 void CAL2DFDCOE_LSM(double* c, double r, double bb, int M, double hzx);  // :236
 void order(int N, float* c);                                        // :526
+int D2T(const char* File_Name, float* V, float* D, int Nx, int Nz, int Nx_d_st, int Nx_d_end,
+        float dx, float dz, float dt);                              // DisToTimeAndTimeToDis1D.cpp:115
+int T2D(const char* File_Name, float* V, float* D, int Nx, int Nz, int Nz_V, int Nx_d_st,
+        int Nx_d_end, float dx, float dz, float dt);                // DisToTimeAndTimeToDis1D.cpp:31
+void phase_correction(float* din, float* dout, int ntr, int nt, float angle);  // phase_correction_ricker_decon.cpp:153
 void velocity(char* OutNameVp, float* v, float* v_2, float* r, float* r_1, float* r_2, int NZ,
               int NX, int NXZ, int N2, float tao, float h, int choice);  // GPU_velocity_real.cpp:6
 
@@ -87,4 +94,26 @@ void ref_segy2trace(const char* buf, float* trace, int ns, int format) {
   segy2trace(buf, trace, ns, format);
 }
 
+// Depth/time conversion writes its result to a file and returns its length.
+// D2T and T2D read t0[i*Nz] without writing it (DisToTimeAndTimeToDis1D.cpp:
+// 43-49, 129-135).  At the reference's image sizes that buffer is a fresh
+// mmap-backed allocation, i.e. zero; glibc's M_PERTURB (allocations filled
+// with the complement of 0xff) gives the small test inputs the same zeros.
+int ref_d2t(const char* path, float* V, float* D, int Nx, int Nz, int st, int end, float dx,
+            float dz, float dt) {
+  mallopt(M_PERTURB, 0xff);
+  int n = D2T(path, V, D, Nx, Nz, st, end, dx, dz, dt);
+  mallopt(M_PERTURB, 0);
+  return n;
+}
+int ref_t2d(const char* path, float* V, float* D, int Nx, int Nz, int Nz_V, int st, int end,
+            float dx, float dz, float dt) {
+  mallopt(M_PERTURB, 0xff);
+  int n = T2D(path, V, D, Nx, Nz, Nz_V, st, end, dx, dz, dt);
+  mallopt(M_PERTURB, 0);
+  return n;
+}
+void ref_phase_correction(float* din, float* dout, int ntr, int nt, float angle) {
+  phase_correction(din, dout, ntr, nt, angle);
+}
 }  // extern "C"

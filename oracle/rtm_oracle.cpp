@@ -116,6 +116,8 @@ struct Engine {
   std::vector<R> B;     // one-way scratch (DFWb)
   std::vector<R> SU, SQ, I1, I2;
   std::vector<R> strips;
+  std::vector<int> snap_slot;  // per step k: snapshot slot or -1
+  std::vector<R> snaps;
   std::vector<float> rec, seis;
   std::vector<R> dseis;
   int kf = 2, kb = -1;
@@ -132,6 +134,15 @@ struct Engine {
     B.assign(NXZ, R(0));
     SU.assign(NXZ, R(0)); SQ.assign(NXZ, R(0)); I1.assign(NXZ, R(0)); I2.assign(NXZ, R(0));
     strips.assign(SS * nslot, R(0));
+    snap_slot.assign(NT, -1);
+    int ns = 0;
+    if (g.snap_every > 0)
+      for (int j = g.snap_every; j + 1 <= NT - 3; j += g.snap_every) {
+        if (j < 2) continue;  // u(0), u(1) come from forward_begin, not a step
+        snap_slot[j] = ns++;
+        snap_slot[j + 1] = ns++;
+      }
+    snaps.assign(NXZ * ns, R(0));
     if (g.record) rec.assign((size_t)g.n * NT, 0.f);
     dseis.assign(g.n > 0 ? g.n : 1, R(0));
   }
@@ -338,6 +349,7 @@ struct Engine {
     hybrid2(F[f2].data(), B.data());
     save_strips(k, F[f2].data());
     record_step(k, F[f2].data());
+    if (snap_slot[k] >= 0) std::copy(F[f2].begin(), F[f2].end(), snaps.begin() + NXZ * snap_slot[k]);
     int t = f0; f0 = f1; f1 = f2; f2 = t;  // Deliver (kernel.cu:820) as a rotation
   }
   int forward_advance(int nsteps) {
@@ -444,6 +456,8 @@ struct Engine {
     BkArgs a{this, wv};
     pool.run(NZ, bk_src_rows, &a);             // BKAdd_EFF (kernel.cu:896)
     { int t = f0; f0 = f1; f1 = f2; f2 = t; }  // Deliver_EFF (kernel.cu:903)
+    if (snap_slot[k] >= 0)  // checkpoint: the reconstructed u(k) is replaced by the saved one
+      std::copy(snaps.begin() + NXZ * snap_slot[k], snaps.begin() + NXZ * (snap_slot[k] + 1), F[f1].begin());
     for (int s = 0; s < g.n; ++s) dseis[s] = (R)seis[(size_t)s * NT + k + 1];  // kernel.cu:905-909
     pool.run(NZ, bk_rcv_rows, &a);             // BKAdd (kernel.cu:913)
     hybrid1(B.data(), Q[q2].data(), Q[q1].data(), Q[q0].data());  // BKHybrid1 (kernel.cu:919)

@@ -67,6 +67,9 @@ typedef struct orc_cfg {
   int record;                   /* record receivers in the forward pass */
   int nthreads;                 /* row-parallel threads per step */
   int strip_slots;              /* strip history slots kept (0 = nt); slot k%strip_slots */
+  int snap_every;               /* checkpoint snapshots of u(j), u(j+1) for j = m*snap_every,
+                                   2 <= j, j+1 <= nt-3; backward replaces the reconstructed
+                                   u(k) by its snapshot (0 = none; not in the reference) */
 } orc_cfg;
 
 typedef struct orc_engine orc_engine;

@@ -266,8 +266,8 @@ extern "C" int rtmh_run(const char* rtm_txt, const rtmh_run_opts* opts) {
   fclose(f);
   // the velocity window beside it (vnew.dat, kernel.cu:1074-1107), padded
   // coordinates; note the ifv = 0 loop starts at NZ_BG, not NZ_BG + N2
+  std::vector<float> vw;
   {
-    std::vector<float> vw;
     auto put_v = [&](int z, int x) {
       vw.push_back((z >= 0 && z < J.d.nz && x >= 0 && x < J.d.nx) ? J.v[(size_t)z * J.d.nx + x] : 0.f);
     };
@@ -283,9 +283,39 @@ extern "C" int rtmh_run(const char* rtm_txt, const rtmh_run_opts* opts) {
     fwrite(vw.data(), sizeof(float), vw.size(), fv);
     fclose(fv);
   }
+  // post-stack chain (kernel.cu:1110-1179): both files are read back as
+  // SGY_N columns of mod_NZ samples, in file order; the image goes to two-way
+  // time, is rotated by `angle` degrees and goes back to depth.
+  const int sgy_n = c.nx_ed - c.nx_bg;
+  if (sgy_n > 0) {
+    const size_t need = (size_t)sgy_n * mnz;
+    std::vector<float> mig(need, 0.f), vdt(need, 0.f);
+    std::copy_n(crop.begin(), std::min(crop.size(), need), mig.begin());
+    std::copy_n(vw.begin(), std::min(vw.size(), need), vdt.begin());
+    auto write_file = [&](const char* name, const std::vector<float>& a) {
+      FILE* fo = fopen((res + name).c_str(), "wb");
+      if (!fo) return set_error(rtmh::RTMH_E_IO, "cannot create %s in %s", name, c.result);
+      fwrite(a.data(), sizeof(float), a.size(), fo);
+      fclose(fo);
+      return (int)rtmh::RTMH_OK;
+    };
+    const int ntime = rtmh_d2t(vdt.data(), mig.data(), sgy_n, mnz, 0, sgy_n - 1, c.hz, c.tao, nullptr, 0);
+    if (ntime < 0) return ntime;
+    std::vector<float> mt((size_t)sgy_n * ntime), mtp((size_t)sgy_n * ntime);
+    rc = rtmh_d2t(vdt.data(), mig.data(), sgy_n, mnz, 0, sgy_n - 1, c.hz, c.tao, mt.data(), (long long)mt.size());
+    if (rc < 0) return rc;
+    if ((rc = write_file("RVSP_Migration_Real_T.dat", mt)) != 0) return rc;
+    rtmh_phase_rotate(mt.data(), mtp.data(), sgy_n, ntime, c.angle);
+    if ((rc = write_file("RVSP_Migration_Real_T_phase.dat", mtp)) != 0) return rc;
+    const int ndep = rtmh_t2d(vdt.data(), mtp.data(), sgy_n, ntime, mnz, 0, sgy_n - 1, c.tao, c.hz, nullptr, 0);
+    if (ndep < 0) return ndep;
+    std::vector<float> md((size_t)sgy_n * ndep);
+    rc = rtmh_t2d(vdt.data(), mtp.data(), sgy_n, ntime, mnz, 0, sgy_n - 1, c.tao, c.hz, md.data(), (long long)md.size());
+    if (rc < 0) return rc;
+    if ((rc = write_file("RVSP_Migration_Real_D.dat", md)) != 0) return rc;
+  }
   // SEG-Y output (kernel.cu:1181-1209): SGY_N traces of mod_NZ samples read
   // sequentially from the cropped image.
-  const int sgy_n = c.nx_ed - c.nx_bg;
   if (sgy_n > 0) {
     std::vector<float> data((size_t)sgy_n * mnz, 0.f), sx(sgy_n), sy(sgy_n), dsr(sgy_n);
     std::copy_n(crop.begin(), std::min(crop.size(), data.size()), data.begin());

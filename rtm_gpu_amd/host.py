@@ -86,6 +86,12 @@ def lib():
                                      C.POINTER(C.c_int), C.c_void_p, C.c_longlong]
         L.rtmh_stack.argtypes = [_fp, _fp, C.c_int, C.c_longlong, C.c_int, _fp, _fp]
         L.rtmh_stack.restype = None
+        L.rtmh_d2t.argtypes = [_fp, _fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                               C.c_void_p, C.c_longlong]
+        L.rtmh_t2d.argtypes = [_fp, _fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                               C.c_float, C.c_void_p, C.c_longlong]
+        L.rtmh_phase_rotate.argtypes = [_fp, _fp, C.c_int, C.c_int, C.c_float]
+        L.rtmh_phase_rotate.restype = None
         L.rtmh_run.argtypes = [C.c_char_p, C.POINTER(RunOpts)]
         _lib = L
     return _lib
@@ -217,6 +223,44 @@ def stack(ups, downs, inorm):
     ill = np.zeros(nI, np.float32)
     lib().rtmh_stack(ups.ravel(), downs.ravel(), ups.shape[0], nI, int(inorm), img, ill)
     return img.reshape(ups.shape[1:]), ill.reshape(ups.shape[1:])
+
+
+def d2t(v, d, dz, dt, x_st=0, x_end=None):
+    """Depth -> two-way time of trace-major v, d ([column][depth sample])."""
+    v = np.ascontiguousarray(v, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    nx, nz = v.shape
+    x_end = nx - 1 if x_end is None else x_end
+    nt = lib().rtmh_d2t(v.ravel(), d.ravel(), nx, nz, x_st, x_end, dz, dt, None, 0)
+    _chk(min(nt, 0), "rtmh_d2t")
+    out = np.zeros((x_end - x_st + 1, nt), np.float32)
+    _chk(min(lib().rtmh_d2t(v.ravel(), d.ravel(), nx, nz, x_st, x_end, dz, dt, out.ctypes.data,
+                            out.size), 0), "rtmh_d2t")
+    return out
+
+
+def t2d(v, d, dtt, dzz, x_st=0, x_end=None):
+    """Two-way time -> depth of trace-major d ([column][time sample]) over the
+    depth velocity v ([column][depth sample])."""
+    v = np.ascontiguousarray(v, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    nx, nzv = v.shape
+    nt = d.shape[1]
+    x_end = nx - 1 if x_end is None else x_end
+    nzo = lib().rtmh_t2d(v.ravel(), d.ravel(), nx, nt, nzv, x_st, x_end, dtt, dzz, None, 0)
+    _chk(min(nzo, 0), "rtmh_t2d")
+    out = np.zeros((x_end - x_st + 1, nzo), np.float32)
+    _chk(min(lib().rtmh_t2d(v.ravel(), d.ravel(), nx, nt, nzv, x_st, x_end, dtt, dzz,
+                            out.ctypes.data, out.size), 0), "rtmh_t2d")
+    return out
+
+
+def phase_rotate(d, angle):
+    """Constant-phase rotation of the traces d ([trace][sample]) by angle degrees."""
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.zeros_like(d)
+    lib().rtmh_phase_rotate(d.ravel(), out.ravel(), d.shape[0], d.shape[1], float(angle))
+    return out
 
 
 def run(rtm_txt, workers=1, batch=8, retries=1, strip_format=0, resume=1, verbose=0,

@@ -66,20 +66,21 @@ def build_case(name="small", mod_nz=40, mod_nx=60, n2=10, nfdmax=4, nfdmin=2, nt
     return dict(name=name, cfg=cfg, model=model, r_u=r_u, r_x=r_x, raw=raw, derived=d)
 
 
-def oracle_engine(case, shot=0, prec=32, record=0, nthreads=1):
+def oracle_engine(case, shot=0, prec=32, record=0, nthreads=1, snap_every=0):
     cfg = dict(case["cfg"])
-    cfg.update(r_u=case["r_u"][shot], r_x=case["r_x"][shot], record=record, nthreads=nthreads)
+    cfg.update(r_u=case["r_u"][shot], r_x=case["r_x"][shot], record=record, nthreads=nthreads,
+               snap_every=snap_every)
     return ob.Engine(cfg, case["model"], prec=prec)
 
 
-def hip_context(case, nshot=None, record=0, strip_format=0, zseg=0):
+def hip_context(case, nshot=None, record=0, strip_format=0, zseg=0, snap_every=0):
     from rtm_gpu_amd import hip
     cfg = case["cfg"]
     nshot = nshot or len(case["r_u"])
     keys = ("nz", "nx", "n2", "nfdmax", "nt", "nt2", "ilste", "icompen", "s_l", "s_r", "s_z",
             "ds", "n", "tao2", "h2", "taoh", "taoh2", "hzx2_1")
     ctx = hip.Context(**{k: cfg[k] for k in keys}, max_shots=nshot, strip_format=strip_format,
-                      record=record, zseg=zseg)
+                      record=record, zseg=zseg, snap_every=snap_every)
     m = case["model"]
     ctx.set_model(m["v"], m["r1"], m["w"], m["c"], m["index"], m["nvel"], m["vmin"], m["dv"])
     ctx.set_wavelet(m["wav"])

@@ -22,7 +22,7 @@ class OrcCfg(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("nz", "nx", "n2", "nfdmax", "nt", "nt2", "ilste",
                                         "icompen", "s_l", "s_r", "s_z", "ds", "n", "r_u", "r_x")] + \
                [(n, C.c_float) for n in ("tao2", "h2", "taoh", "taoh2", "hzx2_1", "vmin", "dv")] + \
-               [(n, C.c_int) for n in ("record", "nthreads", "strip_slots")]
+               [(n, C.c_int) for n in ("record", "nthreads", "strip_slots", "snap_every")]
 
 
 def _build_oracle():
@@ -111,6 +111,13 @@ def ref():
                                     fp, C.c_char_p]
         R.ref_trace2segy.argtypes = [C.c_char_p, fp, C.c_int, C.c_int]
         R.ref_segy2trace.argtypes = [C.c_char_p, fp, C.c_int, C.c_int]
+        R.ref_d2t.argtypes = [C.c_char_p, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                              C.c_float, C.c_float]
+        R.ref_d2t.restype = C.c_int
+        R.ref_t2d.argtypes = [C.c_char_p, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_float, C.c_float, C.c_float]
+        R.ref_t2d.restype = C.c_int
+        R.ref_phase_correction.argtypes = [fp, fp, C.c_int, C.c_int, C.c_float]
         _ref = R
     return _ref
 

@@ -12,6 +12,8 @@ import oracle_bind as ob
 from rtm_gpu_amd import host
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+needs_ref = pytest.mark.skipif(not ob.ref_available(), reason="oracle/_ref not built "
+                               "(reference tree absent)")
 
 
 @pytest.mark.parametrize("args", [
@@ -230,3 +232,93 @@ def test_missing_files_are_clean_errors(tmp_path):
     write_param_files(tmp_path)
     with pytest.raises(host.HostError, match="cannot open depth file|cannot open model"):
         host.run(tmp_path / "2D_Real_RVSP_RTM.txt")
+
+
+# ---- post-stack chain (kernel.cu:1110-1179) ------------------------------------
+def _post_inputs(ncol=23, nz=57, seed=11):
+    rng = np.random.default_rng(seed)
+    base = np.linspace(1500.0, 4200.0, nz, dtype=np.float32)
+    v = (base[None, :] + rng.uniform(-150, 150, (ncol, nz))).astype(np.float32)
+    img = rng.standard_normal((ncol, nz)).astype(np.float32)
+    return v, img
+
+
+def _ref_file(fn, path, *args):
+    n = fn(str(path).encode(), *args)
+    return n, np.fromfile(path, np.float32)
+
+
+@needs_ref
+@pytest.mark.parametrize("hz,tao", [(12.5, 0.001), (20.0, 0.001), (10.0, 0.0008)])
+def test_depth_time_conversion_matches_reference(tmp_path, hz, tao):
+    v, img = _post_inputs()
+    ncol, nz = v.shape
+    t = host.d2t(v, img, hz, tao)
+    n, ref = _ref_file(ob.ref().ref_d2t, tmp_path / "t.dat", v.ravel(), img.ravel(), ncol, nz, 0,
+                       ncol - 1, hz, hz, tao)
+    assert t.shape == (ncol, n)
+    assert np.array_equal(t.ravel().view(np.uint32), ref.view(np.uint32))
+    d = host.t2d(v, t, tao, hz)
+    n2, ref2 = _ref_file(ob.ref().ref_t2d, tmp_path / "d.dat", v.ravel(), t.ravel(), ncol,
+                         t.shape[1], nz, 0, ncol - 1, hz, tao, hz)
+    assert d.shape == (ncol, n2)
+    assert np.array_equal(d.ravel().view(np.uint32), ref2.view(np.uint32))
+    # a column window (the reference indexes the data from its first trace)
+    t3 = host.d2t(v, img, hz, tao, x_st=3, x_end=17)
+    _, ref3 = _ref_file(ob.ref().ref_d2t, tmp_path / "t3.dat", v.ravel(), img.ravel(), ncol, nz,
+                        3, 17, hz, hz, tao)
+    assert np.array_equal(t3.ravel().view(np.uint32), ref3.view(np.uint32))
+
+
+@needs_ref
+@pytest.mark.parametrize("nt,angle", [(300, 90.0), (256, 37.5), (1000, 180.0), (77, 0.0)])
+def test_phase_rotation_matches_reference(nt, angle):
+    rng = np.random.default_rng(nt)
+    d = rng.standard_normal((5, nt)).astype(np.float32)
+    ours = host.phase_rotate(d, angle)
+    ref = np.zeros_like(d)
+    ob.ref().ref_phase_correction(d.ravel(), ref.ravel(), 5, nt, angle)
+    assert np.array_equal(ours.view(np.uint32), ref.view(np.uint32))
+
+
+def test_phase_rotation_known_answers():
+    # 90 degrees on an exact-bin cosine gives -sine (bins 1..n/2-1 rotate; the
+    # DC and Nyquist bins of a cosine are zero).
+    n = 512
+    i = np.arange(n)
+    for f in (3, 40, 200):
+        x = np.cos(2 * np.pi * f * i / n).astype(np.float32)
+        for ang in (90.0, 45.0, 180.0):
+            y = host.phase_rotate(x[None], ang)[0]
+            exp = np.cos(2 * np.pi * f * i / n + np.deg2rad(ang))
+            assert np.max(np.abs(y - exp)) < 2e-6, (f, ang)
+    # 0 degrees leaves bins 1..n/2-1 alone; bins 0 and n/2 re-enter the inverse
+    # transform as |F|/(n/2) with phase atan(0/F) = 0, a closed-form change.
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n)
+    y = host.phase_rotate(x.astype(np.float32)[None], 0.0)[0]
+    xf = x.astype(np.float32).astype(np.float64)
+    f0 = xf.sum()
+    fn = (xf * (-1.0) ** i).sum()
+    exp = xf + (abs(f0) / (n / 2) - f0) / n + (abs(fn) / (n / 2) - fn) * (-1.0) ** i / n
+    assert np.max(np.abs(y - exp)) < 1e-5
+
+
+def test_depth_time_conversion_known_answers():
+    # Constant velocity: two-way time is linear in depth, and linear
+    # interpolation reproduces a depth ramp as a time ramp exactly (to rounding).
+    ncol, nz, hz, tao, vel = 4, 200, 10.0, 0.001, 2000.0
+    v = np.full((ncol, nz), vel, np.float32)
+    z = np.arange(nz, dtype=np.float32) * hz
+    img = np.tile(z, (ncol, 1))
+    t = host.d2t(v, img, hz, tao)
+    nt = t.shape[1]
+    assert nt == int(np.float32(2 * hz / vel) * (nz - 1) / tao) or abs(nt - (nz - 1) * 2 * hz / vel / tao) <= 1
+    tt = np.arange(1, nt) * tao
+    depth_at_t = tt * vel / 2  # the ramp value at two-way time tt
+    assert np.max(np.abs(t[0, 1:] - depth_at_t)) < 1e-2
+    assert t[0, 0] == img[0, 0]
+    # and back to depth: the round trip is the identity away from the ends
+    d = host.t2d(v, t, tao, hz)
+    m = min(d.shape[1], nz) - 2
+    assert np.max(np.abs(d[0, 1:m] - img[0, 1:m])) < 0.05

@@ -329,6 +329,52 @@ def test_time_reversal_reconstruction_fp64():
     assert cases.rel_l2(a, b) < 1e-9
 
 
+def test_snapshot_checkpoints_fp64():
+    # snap_every = 1 checkpoints u(k) for every 2 <= k <= nt-3, so the backward
+    # source field is the forward field at every step (u(1) and u(0) are then
+    # rebuilt from exact neighbours) and the source illumination
+    # I2 = sum_k u(k)^2 follows from the forward fields alone.
+    # Without checkpoints the wavelet terms of kernel.cu:889 (f((k+1)tau) for
+    # k < NT2) make the early reconstructed fields, and I2, differ.
+    case = cases.build_case(name="snap", nt=160)
+    nt = case["cfg"]["nt"]
+    n2 = case["cfg"]["n2"]
+    seis = np.zeros((case["cfg"]["n"], nt), np.float32)
+    e = cases.oracle_engine(case, prec=64)
+    e.forward_begin()
+    fwd = [np.zeros((case["cfg"]["nz"], case["cfg"]["nx"])), e.field(0, f64=True)]
+    for _ in range(nt - 2):
+        e.forward_advance(1)
+        fwd.append(e.field(0, f64=True))
+    expect = sum(u * u for u in fwd[1:])[n2:-n2, n2:-n2]
+
+    def i2_of(snap):
+        cfg = dict(case["cfg"])
+        cfg.update(r_u=case["r_u"][0], r_x=case["r_x"][0], snap_every=snap)
+        eng = ob.Engine(cfg, case["model"], prec=64)
+        eng.forward_begin()
+        eng.forward_advance(nt)
+        eng.backward_begin(seis)
+        eng.backward_advance(nt)
+        return eng.field(4, f64=True)[n2:-n2, n2:-n2]
+
+    err_snap = cases.rel_l2(i2_of(1), expect)
+    err_plain = cases.rel_l2(i2_of(0), expect)
+    assert err_snap < 1e-12, err_snap
+    assert err_plain > 1e3 * max(err_snap, 1e-16), (err_plain, err_snap)
+    # a coarser spacing still restores the checkpointed steps exactly
+    cfg = dict(case["cfg"])
+    cfg.update(r_u=case["r_u"][0], r_x=case["r_x"][0], snap_every=40)
+    eng = ob.Engine(cfg, case["model"], prec=64)
+    eng.forward_begin()
+    eng.forward_advance(nt)
+    eng.backward_begin(seis)
+    eng.backward_advance(nt - 3 - 40)  # steps nt-3 .. 41: the field is now u(41)
+    assert np.array_equal(eng.field(0, f64=True), fwd[41])
+    eng.backward_advance(1)            # u(40)
+    assert np.array_equal(eng.field(0, f64=True), fwd[40])
+
+
 def test_flat_reflector_image_depth():
     # One flat interface; data modelled on the sharp model, migrated on it:
     # the Laplacian-filtered image peaks at the interface depth.
