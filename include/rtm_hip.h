@@ -67,6 +67,11 @@ typedef struct rtm_params {
   int strip_format;      /* RTM_STRIP_F32 (reference), RTM_STRIP_F16, RTM_STRIP_BF16 */
   int record;            /* 1: allocate receiver records for forward modelling */
   int zseg;              /* rows per wave in the interior kernels (0 = auto) */
+  int snap_every;        /* checkpoint snapshots (not in the reference; 0 = none): the
+                            forward pass keeps the whole field u(j) and u(j+1) for
+                            j = m*snap_every, 2 <= j, j+1 <= nt-3, in strip_format with a
+                            per-snapshot power-of-two scale; the backward pass replaces
+                            the reconstructed u(k) by its snapshot */
 } rtm_params;
 
 typedef struct rtm_ctx rtm_ctx;

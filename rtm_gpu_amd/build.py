@@ -21,11 +21,14 @@ ARCH = os.environ.get("RTM_ARCH", "gfx950")
 
 # -ffp-contract=off: every expression is evaluated as written; the kernels
 # place their fused multiply-adds explicitly.  The packed-FP32 feature is
-# switched off (no v_pk_*_f32).  Denormals are kept.
+# switched off (no v_pk_*_f32).  Denormals are kept.  The interior kernel's
+# row loop must unroll fully (its register queue is indexed by the unrolled
+# row), so the size limit of requested unrolling is raised above LLVM's 16K.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang",
     "-packed-fp32-ops", "-Wno-unused-command-line-argument",
+    "-mllvm", "-pragma-unroll-threshold=262144",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", "-Wall",
               "-Wno-unused-result"]

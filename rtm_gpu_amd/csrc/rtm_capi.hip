@@ -34,6 +34,8 @@ struct rtm_ctx {
   Model m{};
   Rcv r{};
   Strips st{};
+  Snaps sn{};
+  std::vector<int> snap_slot;  // per step k: snapshot slot or -1
   int nfd = 0;           // compile-time stencil half-length: longest operator of the table
   int device = 0;
   hipStream_t stream = nullptr;
@@ -41,7 +43,8 @@ struct rtm_ctx {
   size_t bytes = 0;
   std::vector<void*> allocs;
   // device buffers
-  float *d_v = nullptr, *d_r1 = nullptr, *d_ctab = nullptr, *d_w = nullptr;
+  float *d_v = nullptr, *d_r1 = nullptr, *d_ctab = nullptr, *d_w = nullptr, *d_pm = nullptr;
+  unsigned* d_ko = nullptr;
   size_t ctab_n = 0;
   float* F[3] = {nullptr, nullptr, nullptr};
   float* Q[3] = {nullptr, nullptr, nullptr};
@@ -59,7 +62,7 @@ struct rtm_ctx {
   int f0 = 0, f1 = 1, f2 = 2, q0 = 0, q1 = 1, q2 = 2;
   int kf = -1, kb = -2;  // next forward / backward step; -1/-2 = not begun
   bool model_ok = false, wav_ok = false, shots_ok = false, data_ok = false, save_strips = true;
-  int zs = 32, nxt = 1, nblk = 1;
+  int zs = 32, nxt = 1;
   // captured migration (hipGraph)
   bool graph_on = false;
   hipGraph_t graph = nullptr;
@@ -175,8 +178,6 @@ void fill_int_args(rtm_ctx* c, IntArgs& a) {
   a.nt = c->p.nt;
   a.nxt = c->nxt;
   a.zs = c->zs;
-  a.f64comb = c->p.ilste != 0;
-  a.rec = nullptr;
   a.data = c->d_data;
   a.uk = nullptr;
   a.su = c->SU; a.sq = c->SQ; a.i1 = c->I1; a.i2 = c->I2;
@@ -190,7 +191,6 @@ void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
   a.src = c->d_src;
   a.nt = c->p.nt;
   a.f64comb = c->p.ilste != 0;
-  a.rec = nullptr;
   a.data = c->d_data;
   a.st = c->st;
   a.save_strips = 0;
@@ -205,21 +205,24 @@ int forward_step(rtm_ctx* c, int k) {
     fill_int_args(c, a);
     a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
     a.wav = wv; a.k = k;
-    a.rec = c->p.record ? c->d_rec : nullptr;
     Timed t(c, RTM_K_FWD_INTERIOR);
-    HIPCHK(c, launch_interior(EPI_FWD, c->nfd, a, c->nshot, c->nblk, c->stream));
+    // Add (FP32 combine) or Add_Con (FP64 combine) (kernel.cu:46-114)
+    HIPCHK(c, launch_interior(c->p.ilste ? EPI_FWD64 : EPI_FWD, c->nfd, a, c->nshot, c->stream));
   }
   {
     FrameArgs a;
     fill_frame_args(c, a);
     a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
     a.wav = wv; a.k = k;
-    a.rec = c->p.record ? c->d_rec : nullptr;
     a.save_strips = c->save_strips ? 1 : 0;
     Timed t(c, RTM_K_FWD_FRAME);
     HIPCHK(c, launch_frame(FR_FWD, c->nfd, a, c->nshot, c->ntiles, c->stream));
   }
   if (quant && k == c->p.nt - 1) HIPCHK(c, launch_strip_finish(c->g, c->st, k, c->nshot, c->stream));
+  if (c->save_strips && c->snap_slot[k] >= 0)  // checkpoint snapshot of u(k)
+    HIPCHK(c, launch_snap_save(c->g, c->F[c->f2], c->sn, c->snap_slot[k], c->nshot, c->stream));
+  if (c->p.record)  // receiver recording of u(k)
+    HIPCHK(c, launch_record(c->g, c->r, c->F[c->f2], c->d_rec, k, c->p.nt, c->nshot, c->stream));
   int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t;  // Deliver (kernel.cu:820)
   return RTM_OK;
 }
@@ -232,12 +235,14 @@ int backward_step(rtm_ctx* c, int k) {
     a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
     a.wav = wv; a.k = k;
     Timed t(c, RTM_K_SRC_INTERIOR);
-    HIPCHK(c, launch_interior(EPI_SRC, c->nfd, a, c->nshot, c->nblk, c->stream));
+    HIPCHK(c, launch_interior(EPI_SRC, c->nfd, a, c->nshot, c->stream));
   }
   {  // strips of slot k into the rings of u(k): BKEqual of step k-1 (kernel.cu:892)
     Timed t(c, RTM_K_STRIP_RESTORE);
     HIPCHK(c, launch_strip_restore(c->g, c->F[c->f2], c->st, k, c->nshot, c->stream));
   }
+  if (c->snap_slot[k] >= 0)  // checkpoint: the reconstructed u(k) is replaced by its snapshot
+    HIPCHK(c, launch_snap_restore(c->g, c->F[c->f2], c->sn, c->snap_slot[k], c->nshot, c->stream));
   { int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t; }  // Deliver_EFF (kernel.cu:903)
   {  // BKAdd (kernel.cu:913) + Rel_* (kernel.cu:923-930)
     IntArgs a;
@@ -246,7 +251,7 @@ int backward_step(rtm_ctx* c, int k) {
     a.uk = c->F[c->f1];
     a.wav = 0.f; a.k = k;
     Timed t(c, RTM_K_RCV_INTERIOR);
-    HIPCHK(c, launch_interior(EPI_RCV, c->nfd, a, c->nshot, c->nblk, c->stream));
+    HIPCHK(c, launch_interior(EPI_RCV, c->nfd, a, c->nshot, c->stream));
   }
   {  // BKAdd on the frame + BKHybrid1/2 (kernel.cu:913-920)
     FrameArgs a;
@@ -262,13 +267,14 @@ int backward_step(rtm_ctx* c, int k) {
 
 void build_tiles(rtm_ctx* c, std::vector<int4>& t) {
   const Grid& g = c->g;
-  for (int z0 = 0; z0 < g.n2; z0 += 16)
+  // tile shapes must match k_frame: 32 x 64 (top/bottom), 64 x 32 (left/right)
+  for (int z0 = 0; z0 < g.n2; z0 += 32)
     for (int x0 = 0; x0 < g.nx; x0 += 64) t.push_back(make_int4(z0, x0, 0, 0));
-  for (int z0 = g.nz - g.n2; z0 < g.nz; z0 += 16)
+  for (int z0 = g.nz - g.n2; z0 < g.nz; z0 += 32)
     for (int x0 = 0; x0 < g.nx; x0 += 64) t.push_back(make_int4(z0, x0, 1, 0));
   for (int z0 = g.n2; z0 < g.nz - g.n2; z0 += 64) {
-    for (int x0 = 0; x0 < g.n2; x0 += 16) t.push_back(make_int4(z0, x0, 2, 0));
-    for (int x0 = g.nx - g.n2; x0 < g.nx; x0 += 16) t.push_back(make_int4(z0, x0, 3, 0));
+    for (int x0 = 0; x0 < g.n2; x0 += 32) t.push_back(make_int4(z0, x0, 2, 0));
+    for (int x0 = g.nx - g.n2; x0 < g.nx; x0 += 32) t.push_back(make_int4(z0, x0, 3, 0));
   }
 }
 
@@ -284,7 +290,6 @@ int choose_zs(rtm_ctx* c, int nshot) {
     }
   }
   c->zs = zs;
-  c->nblk = c->nxt * (int)((g.mnz + 4 * zs - 1) / (4 * zs));
   return RTM_OK;
 }
 
@@ -317,6 +322,7 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
   else if (p->s_z < 0 || p->s_z >= p->nz || p->s_l < 0 || p->s_r >= p->nx || p->s_r != (p->n - 1) * p->ds + p->s_l)
     rc = fail(c, RTM_E_ARG, "receiver line outside grid");
   else if (p->strip_format < 0 || p->strip_format > 2) rc = fail(c, RTM_E_ARG, "strip_format");
+  else if (p->snap_every < 0) rc = fail(c, RTM_E_ARG, "snap_every=%d < 0", p->snap_every);
   if (rc != RTM_OK) {
     int code = rc;
     delete c;
@@ -336,6 +342,12 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
   g.x0 = x0;
   g.pitch = ((x0 + g.nx + 4) + 63) / 64 * 64;
   g.fstride = (long long)g.nz * g.pitch;
+  if (g.fstride >= (1LL << 28)) {  // buffer offsets of one shot's field stay below 2^30 bytes
+    int code = fail(c, RTM_E_ARG, "grid of %lld padded cells exceeds 2^28", g.fstride);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return code;
+  }
   c->r = Rcv{p->s_l, p->s_r, p->s_z, p->ds, p->n};
 
   const size_t fb = (size_t)p->max_shots * g.fstride;
@@ -348,6 +360,8 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
   if (rc == RTM_OK) rc = dalloc_t(c, &c->I2, fb);
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_v, (size_t)g.fstride);
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_r1, (size_t)g.fstride);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_pm, (size_t)g.fstride);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_ko, (size_t)g.fstride);
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_w, (size_t)g.n2 + 1);
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_src, (size_t)p->max_shots);
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_data, (size_t)p->max_shots * p->nt * p->n);
@@ -362,6 +376,25 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
     rc = dalloc_t(c, &st.scale, (size_t)p->max_shots * st.nslot);
     if (rc == RTM_OK) rc = dalloc_t(c, &st.amax, (size_t)p->max_shots * st.nslot);
     if (rc == RTM_OK) rc = dalloc_t(c, &st.stage, (size_t)p->max_shots * 2 * st.ss);
+  }
+  // checkpoint snapshots: u(j), u(j+1) for j = m*snap_every, 2 <= j, j+1 <= nt-3
+  c->snap_slot.assign(p->nt, -1);
+  Snaps& sn = c->sn;
+  sn.fmt = p->strip_format;
+  sn.nsnap = 0;
+  if (p->snap_every > 0)
+    for (int j = p->snap_every; j + 1 <= p->nt - 3; j += p->snap_every) {
+      if (j < 2) continue;
+      c->snap_slot[j] = sn.nsnap++;
+      c->snap_slot[j + 1] = sn.nsnap++;
+    }
+  if (rc == RTM_OK && sn.nsnap > 0) {
+    const size_t cells = (size_t)g.nz * g.nx;
+    rc = dalloc(c, &sn.base, (size_t)p->max_shots * sn.nsnap * cells * esz);
+    if (rc == RTM_OK && sn.fmt != RTM_STRIP_F32) {
+      rc = dalloc_t(c, &sn.scale, (size_t)p->max_shots * sn.nsnap);
+      if (rc == RTM_OK) rc = dalloc_t(c, &sn.amax, (size_t)p->max_shots * sn.nsnap);
+    }
   }
   const size_t nI = (size_t)g.mnz * g.mnx;
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_up, (size_t)p->max_shots * nI);
@@ -462,16 +495,30 @@ int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, c
   }
   HIPCHK(c, hipMemcpyAsync(c->d_v, pv.data(), pv.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_r1, pr.data(), pr.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  // Per-cell operands of the interior kernel, from the same FP32 expressions:
+  // p = v*v*tao2*h2 (kernel.cu:73) and the velocity bin of kernel.cu:55-56
+  // (FP32 quotient, double + 0.5, truncation), clamped to the table.
+  std::vector<float> pp((size_t)g.fstride);
+  std::vector<unsigned> pk((size_t)g.fstride);
+  for (size_t i = 0; i < pv.size(); ++i) {
+    const float vv = pv[i];
+    pp[i] = vv * vv * c->p.tao2 * c->p.h2;
+    int kb = 0;
+    if (ls) {
+      const float qf = (vv - vmin) / dv;
+      kb = (int)((double)qf + 0.5);
+      kb = std::min(std::max(kb, 0), nrow - 1);
+    }
+    pk[i] = (unsigned)kb * (unsigned)CS * 4u;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_pm, pp.data(), pp.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_ko, pk.data(), pk.size() * sizeof(unsigned), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_w, w, (g.n2 + 1) * sizeof(float), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   Model& m = c->m;
   m.v = c->d_v; m.r1 = c->d_r1; m.ctab = c->d_ctab; m.w = c->d_w;
-  m.cs = CS; m.nrow = nrow; m.ls = ls ? 1 : 0;
-  m.vmin = vmin; m.dv = dv;
-  int ex;
-  float fr = frexpf(dv, &ex);
-  m.dv_pow2 = (fr == 0.5f) ? 1 : 0;
-  m.inv_dv = m.dv_pow2 ? ldexpf(1.0f, 1 - ex) : 1.0f / dv;
+  m.pm = c->d_pm; m.ko = c->d_ko;
+  m.cs = CS; m.nrow = nrow;
   m.hzx2_1 = c->p.hzx2_1; m.tao2 = c->p.tao2; m.h2 = c->p.h2;
   m.taoh = c->p.taoh; m.taoh2 = c->p.taoh2;
   c->nfd = nfd;

@@ -21,10 +21,11 @@ struct Grid {
 struct Model {
   const float* v;
   const float* r1;
+  const float* pm;     // v*v*tao2*h2 per cell (FP32, left to right), padded layout
+  const unsigned* ko;  // byte offset of the cell's coefficient row in ctab, padded layout
   const float* ctab;
   const float* w;
-  int cs, nrow, ls, dv_pow2;
-  float vmin, dv, inv_dv;
+  int cs, nrow;
   float hzx2_1, tao2, h2, taoh, taoh2;
 };
 
@@ -42,7 +43,15 @@ struct Strips {
   long long ss;
 };
 
-enum { EPI_FWD = 0, EPI_SRC = 1, EPI_RCV = 2 };
+// Checkpoint snapshots of whole fields, compact [nz][nx] per (shot, slot).
+struct Snaps {
+  void* base;          // [shot][slot][nz*nx] elements (fp32 / fp16 / bf16)
+  float* scale;        // [shot][slot] (fp16/bf16)
+  unsigned* amax;      // [shot][slot] max |value| bits (fp16/bf16)
+  int fmt, nsnap;
+};
+
+enum { EPI_FWD = 0, EPI_SRC = 1, EPI_RCV = 2, EPI_FWD64 = 3 };
 enum { FR_FWD = 0, FR_RCV = 1 };
 
 struct IntArgs {
@@ -55,9 +64,7 @@ struct IntArgs {
   const int2* src;     // per shot (r_u, r_x)
   float wav;           // source amplitude this step (0 past the gate)
   int k, nt;
-  int nxt, zs;         // interior x tiles, rows per wave
-  int f64comb;         // forward: 1 when ilste = 1 (Add_Con)
-  float* rec;          // [shot][nt][n] or null
+  int nxt, zs;         // interior x tiles, rows per wave (requested)
   const float* data;   // [shot][nt][n] (receiver mode)
   const float* uk;     // u(k) for imaging (receiver mode)
   float *su, *sq, *i1, *i2;
@@ -76,7 +83,6 @@ struct FrameArgs {
   float wav;
   int k, nt;
   int f64comb;
-  float* rec;
   const float* data;
   Strips st;
   int save_strips;
@@ -84,8 +90,10 @@ struct FrameArgs {
 
 // Launchers (rtm_kernels.hip).  nfd_inst is the compile-time stencil
 // half-length the call is dispatched to (>= model nfdmax, <= n2).
-hipError_t launch_interior(int epi, int nfd_inst, const IntArgs& a, int nshot, int nblk,
-                           hipStream_t s);
+// Rows of the interior kernel's register queue; a wave's segment is a
+// multiple of it.
+int interior_queue_rows(int epi, int nfd);
+hipError_t launch_interior(int epi, int nfd_inst, IntArgs a, int nshot, hipStream_t s);
 hipError_t launch_frame(int mode, int nfd_inst, const FrameArgs& a, int nshot, int ntiles,
                         hipStream_t s);
 hipError_t launch_fill(float* p, long long n, float val, hipStream_t s);
@@ -96,6 +104,10 @@ hipError_t launch_strip_save(const Grid& g, const float* f, Strips st, int k, in
 hipError_t launch_strip_restore(const Grid& g, float* f, Strips st, int k, int nshot,
                                 hipStream_t s);
 hipError_t launch_strip_finish(const Grid& g, Strips st, int k, int nshot, hipStream_t s);
+hipError_t launch_snap_save(const Grid& g, const float* f, Snaps sn, int slot, int nshot,
+                            hipStream_t s);
+hipError_t launch_snap_restore(const Grid& g, float* f, Snaps sn, int slot, int nshot,
+                               hipStream_t s);
 hipError_t launch_record(const Grid& g, const Rcv& r, const float* f, float* rec, int k, int nt,
                          int nshot, hipStream_t s);
 hipError_t launch_bwd_begin(const Grid& g, const float* s0, const float* s1, float* q0,

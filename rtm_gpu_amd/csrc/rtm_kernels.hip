@@ -39,15 +39,34 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Velocity bin of kernel.cu:55-56: FP32 quotient, double + 0.5, truncate.
-// Clamped to the table so that guard cells can never index outside it.
-__device__ __forceinline__ int bin_of(float v, const Model& m) {
-  if (!m.ls) return 0;
-  float q = m.dv_pow2 ? (v - m.vmin) * m.inv_dv : (v - m.vmin) / m.dv;
-  int k = (int)((double)q + 0.5);
-  k = k < 0 ? 0 : k;
-  k = k >= m.nrow ? m.nrow - 1 : k;
-  return k;
+// Raw buffer access.  Word 3 of a gfx9-family resource: 32-bit data format,
+// no swizzle, stride 0 (range check on the byte offset against num_records).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int kRsrcWord3 = 0x00020000;
+constexpr unsigned kDrop = 0x80000000u;  // column offset that is out of range of every buffer
+__device__ __forceinline__ Rsrc rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, kRsrcWord3);
+}
+__device__ __forceinline__ float4 bld4(Rsrc r, unsigned vo, unsigned so) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so, 0);
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                     __uint_as_float(v.w));
+}
+__device__ __forceinline__ uint4 bld4u(Rsrc r, unsigned vo, unsigned so) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so, 0);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+// Stores take the whole offset in the VGPR operand and a zero soffset.  A
+// store of more than 64 bits reads its data VGPRs late, so a VALU write to them
+// right after the store needs one wait state; the compiler inserts it only for
+// stores whose soffset is not a register.  (With the row offset in an SGPR
+// soffset the q store of the receiver step lost its first dword in some lanes
+// when two blocks shared a CU.)  vo | kDrop stays >= 2^31 after adding so < 2^30.
+__device__ __forceinline__ void bst4(Rsrc r, unsigned vo, unsigned so, float a, float b, float c,
+                                     float d) {
+  const u32x4 v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c), __float_as_uint(d)};
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(vo + so), 0, 0);
 }
 
 template <int NFD>
@@ -55,10 +74,14 @@ struct CoefRow {
   static constexpr int CS = ((2 + NFD) + 3) & ~3;
   static constexpr int NR = CS / 4;
   float4 r[NR];
-  __device__ __forceinline__ void load(const float* ctab, int kb) {
-    const float4* p = reinterpret_cast<const float4*>(ctab + (long)kb * CS);
+  __device__ __forceinline__ void load(const float* ctab, unsigned byte_off) {
+    const float4* p = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(ctab) + byte_off);
 #pragma unroll
     for (int i = 0; i < NR; ++i) r[i] = p[i];
+  }
+  __device__ __forceinline__ void load(Rsrc t, unsigned byte_off) {
+#pragma unroll
+    for (int i = 0; i < NR; ++i) r[i] = bld4(t, byte_off + 16u * i, 0);
   }
   __device__ __forceinline__ double c0d() const {
     return __hiloint2double(__float_as_int(r[0].y), __float_as_int(r[0].x));
@@ -70,13 +93,11 @@ struct CoefRow {
 // Final combine of the two-way update.
 //  FP32 form (Add, BKAdd, BKAdd_Con: kernel.cu:73,375,413): 2*u1-u0+P.
 //  FP64 form (Add_Con, BKAdd_EFF, BKAdd_EFF_Con: kernel.cu:106,275,312): 2.0*u1-u0+P.
-__device__ __forceinline__ float combine(float u, float u0, float vv, float w1, const Model& m,
-                                         bool f64) {
-  float p = vv * vv * m.tao2 * m.h2;
+// p = v*v*tao2*h2 (evaluated left to right in FP32).
+__device__ __forceinline__ float combine_p(float u, float u0, float p, float w1, bool f64) {
   if (f64) return (float)((2.0 * (double)u - (double)u0) + (double)(p * w1));
   return __builtin_fmaf(p, w1, 2.0f * u - u0);
 }
-
 __device__ __forceinline__ bool is_receiver(int c, const Rcv& r, int* s) {
   if (c < r.s_l || c > r.s_r) return false;
   int d = c - r.s_l;
@@ -84,6 +105,24 @@ __device__ __forceinline__ bool is_receiver(int c, const Rcv& r, int* s) {
   *s = d / r.ds;
   return true;
 }
+
+// Row-major walk of a tile with `cols` columns by a 256-thread block: thread
+// t visits cells t, t+256, ... keeping (row, column) incrementally, so the
+// loop carries no integer division.
+struct TileWalk {
+  int i, j, di, dj, cols;
+  __device__ __forceinline__ explicit TileWalk(int c) : cols(c) {
+    i = (int)threadIdx.x / c;
+    j = (int)threadIdx.x - i * c;
+    di = 256 / c;
+    dj = 256 - di * c;
+  }
+  __device__ __forceinline__ void next() {
+    i += di;
+    j += dj;
+    if (j >= cols) { j -= cols; ++i; }
+  }
+};
 
 __device__ __forceinline__ int mirror_idx(int g, int n) {  // kernel.cu:65-68
   if (g < 0) g = -g;
@@ -94,6 +133,20 @@ __device__ __forceinline__ int mirror_idx(int g, int n) {  // kernel.cu:65-68
 }
 
 // ---- strip encoding -------------------------------------------------------
+// Power-of-two scale of a slot from its max |value|: 2^(e-15) with
+// max = f*2^e, so |value/scale| < 2^15 and FP16 cannot overflow.  The scale is
+// kept >= 2^-126 so that 1/scale stays finite when the maximum is subnormal
+// (slots holding only the tiny precursors of the first steps).
+__device__ __forceinline__ float pow2_scale(float amax) {
+  float sc = 1.0f;
+  if (amax > 0.0f) {
+    int e;
+    frexpf(amax, &e);
+    sc = ldexpf(1.0f, max(e - 15, -126));
+  }
+  return sc;
+}
+
 __device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest even
   unsigned u = __float_as_uint(f);
   if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
@@ -160,170 +213,206 @@ __device__ __forceinline__ float strip_get(const Strips& st, int shot, int k, lo
 // ---------------------------------------------------------------------------
 // Interior update: cells [n2, nz-n2) x [n2, nx-n2).  One wave owns 256
 // columns (4 per lane, float4 rows) and marches down a segment of rows.
-// Vertical taps come from a register queue of rows (row z+NFD+1 is fetched
-// one row ahead); horizontal taps from a per-wave LDS row with a halo of H
-// columns each side.  No mirror is needed: every tap stays inside the grid
-// because NFD <= n2.
+// Vertical taps come from a register queue of rows; horizontal taps from a
+// per-wave LDS row with a halo of H columns each side.  No mirror is needed:
+// every tap stays inside the grid because NFD <= n2.
+//
+// Memory goes through buffer instructions: the per-shot base and size live in
+// a scalar resource, the row offset in a scalar register, the column offset
+// in a per-lane register that stays fixed along the march.  A lane that must
+// not store gets a column offset at or above 2^31, past every buffer's size,
+// and the hardware drops the access; the row body therefore has no branch.
+//
+// Software pipeline per row: (a) the coefficient rows of the row are gathered
+// (their byte offsets were fetched PD rows earlier); (b) every stencil operand
+// of row z+PD is fetched (u1 row z+NFD+PD, the halo row, u0, the p and offset
+// rows), and for the receiver step the imaging operands of row z+1; (c) the
+// LDS exchange and the stencil read only data fetched in earlier iterations
+// or in (a).  The memory counter is in-order, so (a) goes first: the wait on
+// the gathers then leaves the prefetches in flight.  Scheduling barriers keep
+// that order.  A segment is a multiple of the queue length QL, so the row
+// loop is fully unrolled without a remainder; rows past the interior at the
+// end of the last segment are computed from clamped rows and not stored.
+template <int EPI>
+struct EpiTraits {
+  static constexpr bool f64 = EPI == EPI_FWD64 || EPI == EPI_SRC;  // "2.0*" combine
+  static constexpr bool rcv = EPI == EPI_RCV;
+  static constexpr int PD = rcv ? 1 : 2;  // prefetch distance of the stencil operands
+};
+
 template <int NFD, int EPI>
 __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
+  using T = EpiTraits<EPI>;
   constexpr int H = (NFD + 3) & ~3;
-  constexpr int QL = 2 * NFD + 2;
+  constexpr int PD = T::PD;
+  constexpr int QL = 2 * NFD + 1 + PD;
   constexpr int W4 = 64 + 2 * (H / 4);
   constexpr int NWIN = 1 + 2 * (H / 4);
   __shared__ float4 lds[4][W4];
 
   const Grid& g = a.g;
   const Model& m = a.m;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // wave index: uniform in the wave, so row offsets derived from it stay scalar
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int xt = blockIdx.x % a.nxt, zg = blockIdx.x / a.nxt;
   const int shot = blockIdx.y;
   const int zhi = g.nz - g.n2;
-  const int zs = g.n2 + (zg * 4 + wv) * a.zs;
+  const int zs = g.n2 + (zg * 4 + wv) * a.zs;  // a.zs is a multiple of QL
   if (zs >= zhi) return;
-  const int ze = min(zs + a.zs, zhi);
+  const int ze = zs + a.zs;
   const int cend = g.nx - g.n2;                 // first column past the interior
   const int cb = g.n2 + xt * 256;
   const int c0 = cb + lane * 4;
   const int clast = g.nx - ((g.nx - g.n2) & 3); // largest c == n2 (mod 4) with c <= nx
   const int cl = min(c0, clast);
-  const long long so = (long long)shot * g.fstride;
-  const float* __restrict__ U1 = a.u1 + so + g.x0;
-  const float* __restrict__ U0 = a.u0 + so + g.x0;
-  float* __restrict__ U2 = a.u2 + so + g.x0;
-  const float* __restrict__ V = m.v + g.x0;
-  const int P = g.pitch;
-  auto ld = [&](const float* base, int z, int c) -> float4 {
-    return *reinterpret_cast<const float4*>(base + (long)z * P + c);
-  };
+  const unsigned vcol = (unsigned)(g.x0 + cl) * 4u;
+  const unsigned vst = c0 < cend ? (unsigned)(g.x0 + c0) * 4u : kDrop;
+  bool colok[4];
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) colok[jj] = c0 + jj < cend;
 
-  const bool hlane = lane < 2 * (H / 4);
-  int hcol = 0, hpos = 0;
-  if (hlane) {
-    if (lane < H / 4) { hcol = cb - H + 4 * lane; hpos = lane; }
-    else { hcol = cb + 256 + 4 * (lane - H / 4); hpos = 64 + lane; }
-    hcol = min(hcol, clast);
-  }
+  // Halo columns: the first 2H/4 lanes fetch them; the others fetch nothing
+  // (a dropped load returns 0) and write that to their own slot, which the
+  // centre row overwrites right after.
+  unsigned vhal = kDrop;
+  int hpos = H / 4 + lane;
+  if (lane < H / 4) { vhal = (unsigned)(g.x0 + min(cb - H + 4 * lane, clast)) * 4u; hpos = lane; }
+  else if (lane < 2 * (H / 4)) { vhal = (unsigned)(g.x0 + min(cb + 256 + 4 * (lane - H / 4), clast)) * 4u; hpos = 64 + lane; }
+
+  const long long so = (long long)shot * g.fstride;
+  const unsigned fb = (unsigned)g.fstride * 4u;
+  const Rsrc R1 = rsrc(a.u1 + so, fb), R0 = rsrc(a.u0 + so, fb), R2 = rsrc(a.u2 + so, fb);
+  const Rsrc RP = rsrc(m.pm, fb), RK = rsrc(m.ko, fb);
+  const Rsrc RC = rsrc(m.ctab, (unsigned)(m.nrow * m.cs) * 4u);
+  const unsigned P4 = (unsigned)g.pitch * 4u;
+  auto rowb = [&](int z) -> unsigned { return (unsigned)min(z, g.nz - 1) * P4; };
+
   const int2 sp = a.src ? a.src[shot] : make_int2(-1, -1);
+  bool srcc[4];
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) srcc[jj] = c0 + jj == sp.y;
+
+  // receiver step: data replacement of row s_z (kernel.cu:349-354), read once
+  float dval[4] = {0.f, 0.f, 0.f, 0.f};
+  if (T::rcv && a.r.s_z >= zs && a.r.s_z < min(ze, zhi)) {
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      int s;
+      if (colok[jj] && is_receiver(c0 + jj, a.r, &s))
+        dval[jj] = a.data[((long long)shot * a.nt + (a.k + 1)) * a.r.n + s];
+    }
+  }
+  Rsrc RU, RI1, RI2, RSU, RSQ;
+  unsigned vcs = kDrop, vss = kDrop;
+  if (T::rcv) {
+    RU = rsrc(a.uk + so, fb);
+    RI1 = rsrc(a.i1 + so, fb); RI2 = rsrc(a.i2 + so, fb);
+    RSU = rsrc(a.su + so, fb); RSQ = rsrc(a.sq + so, fb);
+    if (a.icompen) { vcs = vcol; vss = vst; }
+  }
 
   float4 q[QL];
 #pragma unroll
-  for (int s = 0; s < 2 * NFD + 1; ++s) q[s] = ld(U1, zs - NFD + s, cl);
+  for (int s = 0; s < 2 * NFD + PD; ++s) q[s] = bld4(R1, vcol, rowb(zs - NFD + s));
+  float4 hvb[PD + 1], u0b[PD + 1], pb[PD + 1];
+  uint4 kb[PD + 1];
+#pragma unroll
+  for (int d = 0; d < PD; ++d) {
+    kb[d] = bld4u(RK, vcol, rowb(zs + d));
+    pb[d] = bld4(RP, vcol, rowb(zs + d));
+    u0b[d] = bld4(R0, vcol, rowb(zs + d));
+    hvb[d] = bld4(R1, vhal, rowb(zs + d));
+  }
+  float4 ukb[2], i1b[2], i2b[2], sub[2], sqb[2];
+  if (T::rcv) {
+    ukb[0] = bld4(RU, vcol, rowb(zs));
+    i1b[0] = bld4(RI1, vcol, rowb(zs));
+    i2b[0] = bld4(RI2, vcol, rowb(zs));
+    sub[0] = bld4(RSU, vcs, rowb(zs));
+    sqb[0] = bld4(RSQ, vcs, rowb(zs));
+  }
 
   for (int z = zs; z < ze; z += QL) {
 #pragma unroll
     for (int j = 0; j < QL; ++j) {
       const int zz = z + j;
-      if (zz < ze) {
-        q[(j + QL - 1) % QL] = ld(U1, min(zz + NFD + 1, g.nz - 1), cl);
-        float4 hv = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (hlane) hv = ld(U1, zz, hcol);
-        const float4 u0v = ld(U0, zz, cl);
-        const float4 vv = ld(V, zz, cl);
-        const float4 qc = q[(j + NFD) % QL];
-        lds[wv][H / 4 + lane] = qc;
-        if (hlane) lds[wv][hpos] = hv;
-        wave_lds_sync();
-        float4 win[NWIN];
+      // (a) coefficient rows of this row
+      CoefRow<NFD> cr[4];
+      cr[0].load(RC, kb[0].x);
+      cr[1].load(RC, kb[0].y);
+      cr[2].load(RC, kb[0].z);
+      cr[3].load(RC, kb[0].w);
+      __builtin_amdgcn_sched_barrier(0);
+      // (b) operands of later rows
+      q[(j + 2 * NFD + PD) % QL] = bld4(R1, vcol, rowb(zz + NFD + PD));
+      kb[PD] = bld4u(RK, vcol, rowb(zz + PD));
+      pb[PD] = bld4(RP, vcol, rowb(zz + PD));
+      u0b[PD] = bld4(R0, vcol, rowb(zz + PD));
+      hvb[PD] = bld4(R1, vhal, rowb(zz + PD));
+      if (T::rcv) {
+        ukb[1] = bld4(RU, vcol, rowb(zz + 1));
+        i1b[1] = bld4(RI1, vcol, rowb(zz + 1));
+        i2b[1] = bld4(RI2, vcol, rowb(zz + 1));
+        sub[1] = bld4(RSU, vcs, rowb(zz + 1));
+        sqb[1] = bld4(RSQ, vcs, rowb(zz + 1));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      // (c) horizontal neighbours through LDS
+      const float4 qc = q[(j + NFD) % QL];
+      lds[wv][hpos] = hvb[0];
+      lds[wv][H / 4 + lane] = qc;
+      wave_lds_sync();
+      float4 win[NWIN];
 #pragma unroll
-        for (int t = 0; t < NWIN; ++t) win[t] = lds[wv][lane + t];
-        wave_lds_sync();
+      for (int t = 0; t < NWIN; ++t) win[t] = lds[wv][lane + t];
+      wave_lds_sync();
 
-        float out[4];
+      const float4 pv = pb[0], u0v = u0b[0];
+      float out[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const float u = compr(qc, jj);
+        float w1 = (float)(cr[jj].c0d() * (double)u);
+#pragma unroll
+        for (int l = 1; l <= NFD; ++l) {
+          float s = compr(q[(j + NFD - l) % QL], jj) + compr(q[(j + NFD + l) % QL], jj);
+          s = __builtin_fmaf(s, m.hzx2_1, compr(win[(H + jj - l) >> 2], (H + jj - l) & 3));
+          s = s + compr(win[(H + jj + l) >> 2], (H + jj + l) & 3);
+          w1 = __builtin_fmaf(cr[jj].c(l), s, w1);
+        }
+        float o = combine_p(u, compr(u0v, jj), compr(pv, jj), w1, T::f64);
+        if (!T::rcv)  // source (kernel.cu:74-77, 277-280)
+          o = (zz == sp.x && srcc[jj]) ? o + a.wav : o;
+        else          // data replacement
+          o = (zz == a.r.s_z && dval[jj] != 0.0f) ? dval[jj] : o;
+        out[jj] = o;
+      }
+      const unsigned rowdrop = zz < zhi ? 0u : kDrop;
+      const unsigned rz = rowb(zz);
+      bst4(R2, vst | rowdrop, rz, out[0], out[1], out[2], out[3]);
+      if (T::rcv) {  // imaging (kernel.cu:489-517); frame columns keep their values
+        const float4 ukv = ukb[0], i1 = i1b[0], i2 = i2b[0], su = sub[0], sq = sqb[0];
+        float r1o[4], r2o[4], rsu[4], rsq[4];
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj) {
-          const float vj = compr(vv, jj);
-          CoefRow<NFD> cr;
-          cr.load(m.ctab, bin_of(vj, m));
-          const float u = compr(qc, jj);
-          float w1 = (float)(cr.c0d() * (double)u);
-#pragma unroll
-          for (int l = 1; l <= NFD; ++l) {
-            float s = compr(q[(j + NFD - l) % QL], jj) + compr(q[(j + NFD + l) % QL], jj);
-            s = __builtin_fmaf(s, m.hzx2_1, compr(win[(H + jj - l) >> 2], (H + jj - l) & 3));
-            s = s + compr(win[(H + jj + l) >> 2], (H + jj + l) & 3);
-            w1 = __builtin_fmaf(cr.c(l), s, w1);
-          }
-          const bool f64 = (EPI == EPI_SRC) || (EPI == EPI_FWD && a.f64comb);
-          out[jj] = combine(u, compr(u0v, jj), vj, w1, m, f64);
+          const float u = compr(ukv, jj);
+          rsu[jj] = a.icompen ? compr(su, jj) + u : u;
+          rsq[jj] = a.icompen ? compr(sq, jj) + out[jj] : out[jj];
+          r1o[jj] = colok[jj] ? __builtin_fmaf(rsq[jj], rsu[jj], compr(i1, jj)) : compr(i1, jj);
+          r2o[jj] = colok[jj] ? __builtin_fmaf(u, u, compr(i2, jj)) : compr(i2, jj);
+          rsu[jj] = colok[jj] ? rsu[jj] : compr(su, jj);
+          rsq[jj] = colok[jj] ? rsq[jj] : compr(sq, jj);
         }
-
-        if (EPI == EPI_FWD || EPI == EPI_SRC) {
-          if (zz == sp.x) {
+        bst4(RI1, vst | rowdrop, rz, r1o[0], r1o[1], r1o[2], r1o[3]);
+        bst4(RI2, vst | rowdrop, rz, r2o[0], r2o[1], r2o[2], r2o[3]);
+        bst4(RSU, vss | rowdrop, rz, rsu[0], rsu[1], rsu[2], rsu[3]);
+        bst4(RSQ, vss | rowdrop, rz, rsq[0], rsq[1], rsq[2], rsq[3]);
+        ukb[0] = ukb[1]; i1b[0] = i1b[1]; i2b[0] = i2b[1]; sub[0] = sub[1]; sqb[0] = sqb[1];
+      }
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
-              if (c0 + jj == sp.y) out[jj] = out[jj] + a.wav;  // kernel.cu:74-77, 277-280
-          }
-        }
-        if (EPI == EPI_RCV && zz == a.r.s_z) {  // data replacement (kernel.cu:349-354)
-#pragma unroll
-          for (int jj = 0; jj < 4; ++jj) {
-            int s;
-            if (c0 + jj < cend && is_receiver(c0 + jj, a.r, &s)) {
-              float d = a.data[((long long)shot * a.nt + (a.k + 1)) * a.r.n + s];
-              if (d != 0.0f) out[jj] = d;
-            }
-          }
-        }
-        float* dst = U2 + (long)zz * P + c0;
-        if (c0 + 3 < cend) {
-          *reinterpret_cast<float4*>(dst) = make_float4(out[0], out[1], out[2], out[3]);
-        } else {
-#pragma unroll
-          for (int jj = 0; jj < 4; ++jj)
-            if (c0 + jj < cend) dst[jj] = out[jj];
-        }
-        if (EPI == EPI_FWD && a.rec && zz == a.r.s_z) {
-#pragma unroll
-          for (int jj = 0; jj < 4; ++jj) {
-            int s;
-            if (c0 + jj < cend && is_receiver(c0 + jj, a.r, &s))
-              a.rec[((long long)shot * a.nt + a.k) * a.r.n + s] = out[jj];
-          }
-        }
-        if (EPI == EPI_RCV) {  // imaging (kernel.cu:489-517), interior only
-          const long long off = so + g.x0 + (long)zz * P + cl;
-          const float4 ukv = *reinterpret_cast<const float4*>(a.uk + off);
-          float4 i1 = *reinterpret_cast<const float4*>(a.i1 + off);
-          float4 i2 = *reinterpret_cast<const float4*>(a.i2 + off);
-          float r1o[4], r2o[4], rsu[4], rsq[4];
-          if (a.icompen) {
-            float4 su = *reinterpret_cast<const float4*>(a.su + off);
-            float4 sq = *reinterpret_cast<const float4*>(a.sq + off);
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-              const float u = compr(ukv, jj);
-              rsu[jj] = compr(su, jj) + u;
-              rsq[jj] = compr(sq, jj) + out[jj];
-              r1o[jj] = __builtin_fmaf(rsq[jj], rsu[jj], compr(i1, jj));
-              r2o[jj] = __builtin_fmaf(u, u, compr(i2, jj));
-            }
-          } else {
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-              const float u = compr(ukv, jj);
-              r1o[jj] = __builtin_fmaf(out[jj], u, compr(i1, jj));
-              r2o[jj] = __builtin_fmaf(u, u, compr(i2, jj));
-              rsu[jj] = rsq[jj] = 0.f;
-            }
-          }
-          const long long o2 = so + g.x0 + (long)zz * P + c0;
-          if (c0 + 3 < cend) {
-            *reinterpret_cast<float4*>(a.i1 + o2) = make_float4(r1o[0], r1o[1], r1o[2], r1o[3]);
-            *reinterpret_cast<float4*>(a.i2 + o2) = make_float4(r2o[0], r2o[1], r2o[2], r2o[3]);
-            if (a.icompen) {
-              *reinterpret_cast<float4*>(a.su + o2) = make_float4(rsu[0], rsu[1], rsu[2], rsu[3]);
-              *reinterpret_cast<float4*>(a.sq + o2) = make_float4(rsq[0], rsq[1], rsq[2], rsq[3]);
-            }
-          } else {
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
-              if (c0 + jj < cend) {
-                a.i1[o2 + jj] = r1o[jj];
-                a.i2[o2 + jj] = r2o[jj];
-                if (a.icompen) { a.su[o2 + jj] = rsu[jj]; a.sq[o2 + jj] = rsq[jj]; }
-              }
-          }
-        }
+      for (int d = 0; d < PD; ++d) {
+        kb[d] = kb[d + 1]; pb[d] = pb[d + 1]; u0b[d] = u0b[d + 1]; hvb[d] = hvb[d + 1];
       }
     }
   }
@@ -331,10 +420,10 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
 
 // ---------------------------------------------------------------------------
 // Frame update (hybrid ABC).  A tile of TZ x TX cells of one band:
-//   band 0 top    rows [0, n2)         (16 x 64 tiles)
-//   band 1 bottom rows [nz-n2, nz)     (16 x 64)
-//   band 2 left   rows [n2, nz-n2), cols [0, n2)      (64 x 16)
-//   band 3 right  rows [n2, nz-n2), cols [nx-n2, nx)  (64 x 16)
+//   band 0 top    rows [0, n2)         (32 x 64 tiles)
+//   band 1 bottom rows [nz-n2, nz)     (32 x 64)
+//   band 2 left   rows [n2, nz-n2), cols [0, n2)      (64 x 32)
+//   band 3 right  rows [n2, nz-n2), cols [nx-n2, nx)  (64 x 32)
 // Phase 1 stages u1 on the tile plus a (1+NFD) halo in LDS with the outer
 // mirror of kernel.cu:65-68 applied at load time.  Phase 2 computes the
 // pre-blend two-way value T (with source / data) on the tile plus a 1-cell
@@ -342,8 +431,8 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
 // (Hybrid1, kernel.cu:116-158) and blends (Hybrid2, kernel.cu:160-183).
 template <int NFD, int MODE>
 __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
-  constexpr int RMAX = (16 + 2 + 2 * NFD) * (64 + 2 + 2 * NFD);
-  constexpr int TMAX = (16 + 2) * (64 + 2);
+  constexpr int RMAX = (32 + 2 + 2 * NFD) * (64 + 2 + 2 * NFD);
+  constexpr int TMAX = (32 + 2) * (64 + 2);
   __shared__ float su1[RMAX];
   __shared__ float sT[TMAX];
   const Grid& g = a.g;
@@ -351,7 +440,7 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   const int4 td = a.tiles[blockIdx.x];
   const int shot = blockIdx.y;
   const int band = td.z;
-  const int TZ = band < 2 ? 16 : 64, TX = band < 2 ? 64 : 16;
+  const int TZ = band < 2 ? 32 : 64, TX = band < 2 ? 64 : 32;
   const int RZ = TZ + 2 + 2 * NFD, RX = TX + 2 + 2 * NFD;
   const int TRZ = TZ + 2, TRX = TX + 2;
   const int z0 = td.x, x0 = td.y;
@@ -364,23 +453,23 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   const int P = g.pitch;
   const int2 sp = a.src ? a.src[shot] : make_int2(-1, -1);
 
-  for (int idx = threadIdx.x; idx < RZ * RX; idx += 256) {
-    int i = idx / RX, jx = idx - i * RX;
+  for (TileWalk t(RX); t.i < RZ; t.next()) {
+    const int i = t.i, jx = t.j, idx = i * RX + jx;
     int gz = mirror_idx(z0 - 1 - NFD + i, g.nz);
     int gx = mirror_idx(x0 - 1 - NFD + jx, g.nx);
     su1[idx] = U1[(long)gz * P + gx];
   }
   __syncthreads();
 
-  for (int idx = threadIdx.x; idx < TRZ * TRX; idx += 256) {
-    int i = idx / TRX, jx = idx - i * TRX;
+  for (TileWalk t(TRX); t.i < TRZ; t.next()) {
+    const int i = t.i, jx = t.j, idx = i * TRX + jx;
     int gz = z0 - 1 + i, gx = x0 - 1 + jx;
     float T = 0.f;
     if (gz >= 0 && gz < g.nz && gx >= 0 && gx < g.nx) {
       const int ci = (i + NFD) * RX + (jx + NFD);
-      const float vj = V[(long)gz * P + gx];
+      const long long pc = (long)gz * P + gx;
       CoefRow<NFD> cr;
-      cr.load(m.ctab, bin_of(vj, m));
+      cr.load(m.ctab, m.ko[g.x0 + pc]);
       const float u = su1[ci];
       float w1 = (float)(cr.c0d() * (double)u);
 #pragma unroll
@@ -390,9 +479,9 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
         s = s + su1[ci + l];
         w1 = __builtin_fmaf(cr.c(l), s, w1);
       }
-      const float u0 = U0[(long)gz * P + gx];
+      const float u0 = U0[pc];
       if (MODE == FR_FWD) {
-        T = combine(u, u0, vj, w1, m, a.f64comb != 0);
+        T = combine_p(u, u0, m.pm[g.x0 + pc], w1, a.f64comb != 0);
         if (gz == sp.x && gx == sp.y) T = T + a.wav;
       } else {
         int s;
@@ -401,15 +490,15 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
           float d = a.data[((long long)shot * a.nt + (a.k + 1)) * a.r.n + s];
           if (d != 0.0f) { T = d; inj = true; }
         }
-        if (!inj) T = combine(u, u0, vj, w1, m, false);
+        if (!inj) T = combine_p(u, u0, m.pm[g.x0 + pc], w1, false);
       }
     }
     sT[idx] = T;
   }
   __syncthreads();
 
-  for (int idx = threadIdx.x; idx < TZ * TX; idx += 256) {
-    const int i = idx / TX, jx = idx - i * TX;
+  for (TileWalk t(TX); t.i < TZ; t.next()) {
+    const int i = t.i, jx = t.j;
     const int z = z0 + i, x = x0 + jx;
     bool in;
     if (band == 0) in = z < g.n2 && x < g.nx;
@@ -440,8 +529,9 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
       else {
         dzi = 0; lz = 1; lx = 0;
         dxi = (x == ar) ? 1 : -1;
-        const long long lin = (long long)ar * g.nx + z;  // Dv[(N2-y)*NX+x], linear index
-        const int zq = (int)(lin / g.nx), xq = (int)(lin % g.nx);
+        // Dv[(N2-y)*NX+x] as a linear index: row ar + z/nx, column z%nx
+        int zq = ar, xq = z;
+        if (xq >= g.nx) { zq += xq / g.nx; xq -= (xq / g.nx) * g.nx; }
         vq = V[(long)zq * P + xq];
       }
       const long long pi = pg + (long)dzi * P + dxi;
@@ -466,9 +556,6 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
         long long e = strip_index(g, z, x);
         if (e >= 0) strip_put(a.st, shot, a.k, e, val);
       }
-      int s;
-      if (a.rec && z == a.r.s_z && is_receiver(x, a.r, &s))
-        a.rec[((long long)shot * a.nt + a.k) * a.r.n + s] = val;
     }
   }
 }
@@ -507,18 +594,11 @@ __global__ void k_strip_save(Grid g, const float* f, Strips st, int k) {
   }
 }
 
-// Quantise the staged fp32 strips of slot k with a power-of-two scale derived
-// from the slot's max |value| (fp16: |value/scale| <= 2^15 < 65504).
+// Quantise the staged fp32 strips of slot k with the slot's power-of-two scale.
 __global__ void k_strip_finish(Grid g, Strips st, int k) {
   const int shot = blockIdx.y;
   const long long slot = (long long)shot * st.nslot + (k % st.nslot);
-  const float amax = __uint_as_float(st.amax[slot]);
-  float sc = 1.0f;
-  if (amax > 0.0f) {
-    int e;
-    frexpf(amax, &e);              // amax = f * 2^e, f in [0.5, 1)
-    sc = ldexpf(1.0f, e - 15);     // amax / sc < 2^15
-  }
+  const float sc = pow2_scale(__uint_as_float(st.amax[slot]));
   if (blockIdx.x == 0 && threadIdx.x == 0) st.scale[slot] = sc;
   const float inv = 1.0f / sc;     // exact: sc is a power of two
   const float* s = st.stage + ((long long)shot * 2 + (k & 1)) * st.ss;
@@ -538,6 +618,72 @@ __global__ void k_strip_restore(Grid g, float* f, Strips st, int k) {
     int z, x;
     strip_cell(g, e, &z, &x);
     F[(long)z * g.pitch + x] = strip_get(st, shot, k, e);
+  }
+}
+
+// ---- checkpoint snapshots -------------------------------------------------
+
+__global__ void k_snap_clear(Snaps sn, int slot, int nshot) {  // a kernel, not a memset: capturable
+  const int sh = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sh < nshot) sn.amax[(long long)sh * sn.nsnap + slot] = 0u;
+}
+
+__global__ void k_snap_amax(Grid g, const float* f, Snaps sn, int slot) {
+  const int shot = blockIdx.y;
+  const float* F = f + (long long)shot * g.fstride + g.x0;
+  const long long n = (long long)g.nz * g.nx;
+  float mx = 0.0f;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int z = (int)(e / g.nx), x = (int)(e % g.nx);
+    mx = fmaxf(mx, fabsf(F[(long)z * g.pitch + x]));
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(&sn.amax[(long long)shot * sn.nsnap + slot], __float_as_uint(mx));
+}
+
+__global__ void k_snap_save(Grid g, const float* f, Snaps sn, int slot) {
+  const int shot = blockIdx.y;
+  const float* F = f + (long long)shot * g.fstride + g.x0;
+  const long long n = (long long)g.nz * g.nx;
+  const long long so = (long long)shot * sn.nsnap + slot;
+  float inv = 1.0f;
+  if (sn.fmt != 0) {
+    const float sc = pow2_scale(__uint_as_float(sn.amax[so]));
+    if (blockIdx.x == 0 && threadIdx.x == 0) sn.scale[so] = sc;
+    inv = 1.0f / sc;  // exact: sc is a power of two
+  }
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int z = (int)(e / g.nx), x = (int)(e % g.nx);
+    const float v = F[(long)z * g.pitch + x];
+    if (sn.fmt == 0) {
+      ((float*)sn.base)[so * n + e] = v;
+    } else {
+      const float y = v * inv;
+      ((unsigned short*)sn.base)[so * n + e] =
+          (sn.fmt == 1) ? __half_as_ushort(__float2half_rn(y)) : f2bf(y);
+    }
+  }
+}
+
+__global__ void k_snap_restore(Grid g, float* f, Snaps sn, int slot) {
+  const int shot = blockIdx.y;
+  float* F = f + (long long)shot * g.fstride + g.x0;
+  const long long n = (long long)g.nz * g.nx;
+  const long long so = (long long)shot * sn.nsnap + slot;
+  const float sc = sn.fmt != 0 ? sn.scale[so] : 1.0f;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int z = (int)(e / g.nx), x = (int)(e % g.nx);
+    float v;
+    if (sn.fmt == 0) {
+      v = ((const float*)sn.base)[so * n + e];
+    } else {
+      const unsigned short h = ((const unsigned short*)sn.base)[so * n + e];
+      v = (sn.fmt == 1 ? __half2float(__ushort_as_half(h)) : bf2f(h)) * sc;
+    }
+    F[(long)z * g.pitch + x] = v;
   }
 }
 
@@ -652,13 +798,21 @@ __global__ void k_stack_finish(int nI, const float* s1, const float* s2, int nre
 // launchers
 #define RTM_NFD_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
-hipError_t launch_interior(int epi, int nfd, const IntArgs& a, int nshot, int nblk, hipStream_t s) {
-  dim3 grid(nblk, nshot), block(256);
-#define CASE(N)                                                                          \
-  case N:                                                                                \
-    if (epi == EPI_FWD) hipLaunchKernelGGL((k_interior<N, EPI_FWD>), grid, block, 0, s, a); \
+int interior_queue_rows(int epi, int nfd) { return 2 * nfd + 1 + (epi == EPI_RCV ? 1 : 2); }
+
+hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s) {
+  // rows per wave: the requested segment rounded to a multiple of the queue length
+  const int ql = interior_queue_rows(epi, nfd);
+  const int mult = a.zs > ql ? (a.zs + ql / 2) / ql : 1;
+  a.zs = mult * ql;
+  const int nseg = (a.g.mnz + a.zs - 1) / a.zs;
+  dim3 grid(a.nxt * ((nseg + 3) / 4), nshot), block(256);
+#define CASE(N)                                                                             \
+  case N:                                                                                   \
+    if (epi == EPI_FWD) hipLaunchKernelGGL((k_interior<N, EPI_FWD>), grid, block, 0, s, a);     \
+    else if (epi == EPI_FWD64) hipLaunchKernelGGL((k_interior<N, EPI_FWD64>), grid, block, 0, s, a); \
     else if (epi == EPI_SRC) hipLaunchKernelGGL((k_interior<N, EPI_SRC>), grid, block, 0, s, a); \
-    else hipLaunchKernelGGL((k_interior<N, EPI_RCV>), grid, block, 0, s, a);            \
+    else hipLaunchKernelGGL((k_interior<N, EPI_RCV>), grid, block, 0, s, a);               \
     break;
   switch (nfd) {
     RTM_NFD_CASES(CASE)
@@ -711,6 +865,22 @@ hipError_t launch_strip_finish(const Grid& g, Strips st, int k, int nshot, hipSt
 hipError_t launch_strip_restore(const Grid& g, float* f, Strips st, int k, int nshot,
                                 hipStream_t s) {
   hipLaunchKernelGGL(k_strip_restore, dim3(blocks_for(st.ss), nshot), dim3(256), 0, s, g, f, st, k);
+  return hipGetLastError();
+}
+hipError_t launch_snap_save(const Grid& g, const float* f, Snaps sn, int slot, int nshot,
+                            hipStream_t s) {
+  const long long n = (long long)g.nz * g.nx;
+  if (sn.fmt != 0) {
+    hipLaunchKernelGGL(k_snap_clear, dim3(blocks_for(nshot)), dim3(256), 0, s, sn, slot, nshot);
+    hipLaunchKernelGGL(k_snap_amax, dim3(blocks_for(n, 256, 512), nshot), dim3(256), 0, s, g, f, sn, slot);
+  }
+  hipLaunchKernelGGL(k_snap_save, dim3(blocks_for(n, 256, 1024), nshot), dim3(256), 0, s, g, f, sn, slot);
+  return hipGetLastError();
+}
+hipError_t launch_snap_restore(const Grid& g, float* f, Snaps sn, int slot, int nshot,
+                               hipStream_t s) {
+  const long long n = (long long)g.nz * g.nx;
+  hipLaunchKernelGGL(k_snap_restore, dim3(blocks_for(n, 256, 1024), nshot), dim3(256), 0, s, g, f, sn, slot);
   return hipGetLastError();
 }
 hipError_t launch_record(const Grid& g, const Rcv& r, const float* f, float* rec, int k, int nt,

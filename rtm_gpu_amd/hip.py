@@ -25,7 +25,8 @@ class RtmParams(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("nz", "nx", "n2", "nfdmax", "nt", "nt2", "ilste",
                                         "icompen", "s_l", "s_r", "s_z", "ds", "n")] + \
                [(n, C.c_float) for n in ("tao2", "h2", "taoh", "taoh2", "hzx2_1")] + \
-               [(n, C.c_int) for n in ("max_shots", "strip_format", "record", "zseg")]
+               [(n, C.c_int) for n in ("max_shots", "strip_format", "record", "zseg",
+                                       "snap_every")]
 
 
 class RtmError(RuntimeError):

@@ -75,6 +75,23 @@ def test_driver_end_to_end(tmp_path, workers, batch):
     data, fmt = host.read_segy(tmp_path / "RVSP_migration_Real.sgy")
     assert fmt == 1 and data.shape == (42, 34)
     assert np.allclose(data, exp.reshape(42, 34), rtol=2 ** -20, atol=1e-30)
+    # post-stack chain (kernel.cu:1110-1179): both files reread as 42 columns of
+    # mod_NZ samples, to time, rotated by `angle`, back to depth
+    vw = np.fromfile(tmp_path / "vnew.dat", np.float32)[:42 * 34].reshape(42, 34)
+    mig = crop[:42 * 34].reshape(42, 34)
+    t = host.d2t(vw, mig, vals["hz"], vals["tao"])
+    assert np.array_equal(np.fromfile(tmp_path / "RVSP_Migration_Real_T.dat", np.float32),
+                          t.ravel())
+    tp = host.phase_rotate(t, vals["angle"])
+    assert np.array_equal(np.fromfile(tmp_path / "RVSP_Migration_Real_T_phase.dat", np.float32),
+                          tp.ravel())
+    dd = host.t2d(vw, tp, vals["tao"], vals["hz"])
+    assert np.array_equal(np.fromfile(tmp_path / "RVSP_Migration_Real_D.dat", np.float32),
+                          dd.ravel())
+    if ob.ref_available():  # and the reference helpers on the same files
+        ref_t = np.zeros_like(tp)
+        ob.ref().ref_phase_correction(t.ravel(), ref_t.ravel(), 42, t.shape[1], vals["angle"])
+        assert np.array_equal(ref_t, tp)
 
 
 def test_driver_resume_skips_done_shots(tmp_path):
@@ -131,3 +148,87 @@ def test_reduced_precision_strips(fmt, tol):
     print(f"strip format {fmt}: image rel L2 vs FP32 strips {err:.2e}, vs FP64 oracle "
           f"{cases.rel_l2(res[fmt], ref64):.2e} (FP32 strips vs FP64: {cases.rel_l2(res[0], ref64):.2e})")
     assert err < tol
+
+
+def test_snapshot_checkpoints_fp32_exact():
+    """Checkpoint snapshots stored in FP32: the backward pass replaces u(k) by
+    the saved forward field at the checkpointed steps, exactly as the oracle
+    does, so every field equals the FP32 replica run with the same spacing."""
+    from rtm_gpu_amd import hip
+    case = cases.build_case(name="ls_nfd4")
+    seis = cases.synth_gather(case)
+    nt = case["cfg"]["nt"]
+    ctx = cases.hip_context(case, nshot=1, snap_every=50)
+    ctx.set_gathers(seis[None])
+    eng = cases.oracle_engine(case, snap_every=50)
+    ctx.forward_begin()
+    ctx.forward_advance(nt)
+    eng.forward_begin()
+    eng.forward_advance(nt)
+    ctx.backward_begin()
+    eng.backward_begin(seis)
+    n2 = case["cfg"]["n2"]
+    done = 0
+    for step in (nt - 3 - 101, 1, 1, 49, nt):  # stop at u(102), u(101), u(100), u(51), end
+        ctx.backward_advance(step)
+        eng.backward_advance(step)
+        done += step
+        g = ctx.field(hip.FIELD_U)[n2:-n2, n2:-n2]
+        assert np.array_equal(g, eng.field(0)[n2:-n2, n2:-n2]), f"u after {done} steps"
+    for which, ow in ((hip.FIELD_I1, 3), (hip.FIELD_I2, 4)):
+        assert np.array_equal(ctx.field(which), eng.field(ow))
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_snapshot_checkpoints_reduced_precision(fmt):
+    """FP16 / BF16 checkpoint snapshots (with FP16 / BF16 strips): the image
+    stays close to FP32 storage; errors against the FP64 oracle run with the
+    same checkpoints are printed."""
+    from rtm_gpu_amd import hip
+    case = cases.build_case(name="strips16", nt=300)
+    seis = cases.synth_gather(case)
+    nt = case["cfg"]["nt"]
+    res = {}
+    for f in (0, fmt):
+        ctx = cases.hip_context(case, nshot=1, strip_format=f, snap_every=100)
+        ctx.set_gathers(seis[None])
+        ctx.migrate()
+        res[f] = ctx.field(hip.FIELD_I1)
+    e64 = cases.oracle_engine(case, prec=64, snap_every=100)
+    e64.forward_begin()
+    e64.forward_advance(nt)
+    e64.backward_begin(seis)
+    e64.backward_advance(nt)
+    ref64 = e64.field(3, f64=True)
+    err = cases.rel_l2(res[fmt], res[0])
+    print(f"snapshots+strips format {fmt}: image rel L2 vs FP32 storage {err:.2e}, vs FP64 "
+          f"oracle {cases.rel_l2(res[fmt], ref64):.2e} (FP32 vs FP64: "
+          f"{cases.rel_l2(res[0], ref64):.2e})")
+    assert err < (2e-3 if fmt == 1 else 1e-2)
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_reduced_precision_subnormal_slots_stay_finite(fmt):
+    """A wavelet scaled by 1e-38 puts every strip and snapshot maximum in the
+    subnormal range: the per-slot power-of-two scale must keep 1/scale finite
+    (it is clamped at 2^-126), so no stored value becomes inf or NaN."""
+    from rtm_gpu_amd import hip
+    case = dict(cases.build_case(name="ls_nfd4"))
+    m = dict(case["model"])
+    m["wav"] = (m["wav"].astype(np.float64) * 1e-38).astype(np.float32)
+    case["model"] = m
+    seis = cases.synth_gather(case)
+    ctx = cases.hip_context(case, nshot=1, strip_format=fmt, snap_every=60)
+    ctx.set_gathers(seis[None])
+    ctx.forward_begin()
+    ctx.forward_advance(case["cfg"]["nt"])
+    k = case["cfg"]["nt"] - 1
+    strip = ctx.strip(0, k)
+    assert np.isfinite(strip).all() and np.abs(strip).max() > 0
+    ctx.backward_begin()
+    ctx.backward_advance(case["cfg"]["nt"] - 50)
+    u = ctx.field(hip.FIELD_U)
+    assert np.isfinite(u).all() and np.abs(u).max() > 0
+    ctx.backward_advance(case["cfg"]["nt"])
+    for which in (hip.FIELD_I1, hip.FIELD_I2, hip.FIELD_Q):
+        assert np.isfinite(ctx.field(which)).all(), f"field {which}"

@@ -132,6 +132,69 @@ def test_backward_and_images(key):
           f"vs FP64 oracle {rel_l2(i1, e64.field(3, f64=True)):.2e}")
 
 
+@pytest.mark.parametrize("zseg", [1, 7, 40])
+def test_ragged_width_and_segments(zseg):
+    """Interior width not a multiple of 4 (the last lane's float4 straddles
+    the right frame) and row segments of several lengths (the interior kernel
+    rounds them to its queue length; the last segment ends past the interior).
+    All fields, frame cells included, must equal the FP32 replica."""
+    from rtm_gpu_amd import hip
+    case = build_case(name="ragged", mod_nz=37, mod_nx=61, nfdmax=3, n2=9, seed=5, nt=200,
+                      src_x1=17)
+    assert (case["cfg"]["nx"] - 2 * case["cfg"]["n2"]) % 4 == 1
+    seis = synth_gather(case)
+    ctx = hip_context(case, nshot=1, zseg=zseg)
+    ctx.set_gathers(seis[None])
+    eng = oracle_engine(case)
+    nt = case["cfg"]["nt"]
+    ctx.forward_begin()
+    ctx.forward_advance(nt)
+    eng.forward_begin()
+    eng.forward_advance(nt)
+    assert np.array_equal(ctx.field(hip.FIELD_U), eng.field(0))
+    ctx.backward_begin()
+    eng.backward_begin(seis)
+    ctx.backward_advance(nt - 2)
+    eng.backward_advance(nt - 2)
+    for which, ow in ((hip.FIELD_Q, 2), (hip.FIELD_I1, 3), (hip.FIELD_I2, 4), (hip.FIELD_SU, 5),
+                      (hip.FIELD_SQ, 6)):
+        g = ctx.field(which)
+        o = eng.field(ow)
+        assert np.array_equal(g, o), f"field {which}: max |diff| {np.max(np.abs(g - o))}"
+
+
+def test_many_identical_shots_agree():
+    """64 copies of one shot on a 200 x 1100 interior: several hundred blocks
+    per launch, so CUs hold more than one block at a time.  Every copy must
+    equal the FP32 replica bit for bit (regression: a wide store whose data
+    registers were overwritten one instruction later lost dwords only under
+    that load)."""
+    from rtm_gpu_amd import hip
+    case = build_case(name="wide", mod_nz=200, mod_nx=1100, n2=12, nfdmax=4, nt=300, seed=6,
+                      src_x1=550, n=1100)
+    seis = synth_gather(case)
+    ns = 64
+    m = case["model"]
+    cfg = case["cfg"]
+    keys = ("nz", "nx", "n2", "nfdmax", "nt", "nt2", "ilste", "icompen", "s_l", "s_r", "s_z",
+            "ds", "n", "tao2", "h2", "taoh", "taoh2", "hzx2_1")
+    ctx = hip.Context(**{k: cfg[k] for k in keys}, max_shots=ns)
+    ctx.set_model(m["v"], m["r1"], m["w"], m["c"], m["index"], m["nvel"], m["vmin"], m["dv"])
+    ctx.set_wavelet(m["wav"])
+    ctx.set_shots(case["r_u"] * ns, case["r_x"] * ns)
+    ctx.set_gathers(np.repeat(seis[None], ns, axis=0))
+    ctx.migrate()
+    eng = oracle_engine(case, nthreads=8)
+    eng.forward_begin()
+    eng.forward_advance(cfg["nt"])
+    eng.backward_begin(seis)
+    eng.backward_advance(cfg["nt"])
+    for which, ow in ((hip.FIELD_I1, 3), (hip.FIELD_I2, 4), (hip.FIELD_Q, 2)):
+        ref = eng.field(ow)
+        bad = [s for s in range(ns) if not np.array_equal(ctx.field(which, shot=s), ref)]
+        assert not bad, f"field {which}: shots {bad[:8]} differ from the replica"
+
+
 def test_batched_shots_equal_single():
     case = build_case(name="batch3", nshot=3, depth_step=40.0)
     seis = np.stack([synth_gather(case, s) for s in range(3)])
