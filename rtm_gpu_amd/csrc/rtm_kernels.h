@@ -65,6 +65,7 @@ struct IntArgs {
   float wav;           // source amplitude this step (0 past the gate)
   int k, nt;
   int nxt, zs;         // interior x tiles, rows per wave (requested)
+  int nshot;           // shots of the launch (the fastest-varying block index)
   const float* data;   // [shot][nt][n] (receiver mode)
   const float* uk;     // u(k) for imaging (receiver mode)
   float *su, *sq, *i1, *i2;
@@ -83,6 +84,7 @@ struct FrameArgs {
   float wav;
   int k, nt;
   int f64comb;
+  int nshot;           // shots of the launch (the fastest-varying block index)
   const float* data;
   Strips st;
   int save_strips;
@@ -94,7 +96,7 @@ struct FrameArgs {
 // multiple of it.
 int interior_queue_rows(int epi, int nfd);
 hipError_t launch_interior(int epi, int nfd_inst, IntArgs a, int nshot, hipStream_t s);
-hipError_t launch_frame(int mode, int nfd_inst, const FrameArgs& a, int nshot, int ntiles,
+hipError_t launch_frame(int mode, int nfd_inst, FrameArgs a, int nshot, int ntiles,
                         hipStream_t s);
 hipError_t launch_fill(float* p, long long n, float val, hipStream_t s);
 hipError_t launch_fwd_begin(const Grid& g, float* f0, float* f1, const int2* src, float spike,

@@ -255,8 +255,13 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   const int lane = threadIdx.x & 63;
   // wave index: uniform in the wave, so row offsets derived from it stay scalar
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int xt = blockIdx.x % a.nxt, zg = blockIdx.x / a.nxt;
-  const int shot = blockIdx.y;
+  // Block b covers shot b % nshot of tile b / nshot.  Workgroups reach the
+  // eight XCDs round-robin by b, so the blocks one XCD holds at a time are
+  // many shots of the same few tiles, and their reads of the shared model
+  // rows (p, coefficient offsets, coefficient rows) hit that XCD's L2.
+  const int shot = (int)(blockIdx.x % (unsigned)a.nshot);
+  const int tile = (int)(blockIdx.x / (unsigned)a.nshot);
+  const int xt = tile % a.nxt, zg = tile / a.nxt;
   const int zhi = g.nz - g.n2;
   const int zs = g.n2 + (zg * 4 + wv) * a.zs;  // a.zs is a multiple of QL
   if (zs >= zhi) return;
@@ -437,8 +442,9 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   __shared__ float sT[TMAX];
   const Grid& g = a.g;
   const Model& m = a.m;
-  const int4 td = a.tiles[blockIdx.x];
-  const int shot = blockIdx.y;
+  // shot-fastest block order, as in k_interior
+  const int shot = (int)(blockIdx.x % (unsigned)a.nshot);
+  const int4 td = a.tiles[blockIdx.x / (unsigned)a.nshot];
   const int band = td.z;
   const int TZ = band < 2 ? 32 : 64, TX = band < 2 ? 64 : 32;
   const int RZ = TZ + 2 + 2 * NFD, RX = TX + 2 + 2 * NFD;
@@ -806,7 +812,8 @@ hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s
   const int mult = a.zs > ql ? (a.zs + ql / 2) / ql : 1;
   a.zs = mult * ql;
   const int nseg = (a.g.mnz + a.zs - 1) / a.zs;
-  dim3 grid(a.nxt * ((nseg + 3) / 4), nshot), block(256);
+  a.nshot = nshot;
+  dim3 grid(a.nxt * ((nseg + 3) / 4) * nshot), block(256);
 #define CASE(N)                                                                             \
   case N:                                                                                   \
     if (epi == EPI_FWD) hipLaunchKernelGGL((k_interior<N, EPI_FWD>), grid, block, 0, s, a);     \
@@ -822,8 +829,9 @@ hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_frame(int mode, int nfd, const FrameArgs& a, int nshot, int ntiles, hipStream_t s) {
-  dim3 grid(ntiles, nshot), block(256);
+hipError_t launch_frame(int mode, int nfd, FrameArgs a, int nshot, int ntiles, hipStream_t s) {
+  a.nshot = nshot;
+  dim3 grid(ntiles * nshot), block(256);
 #define CASE(N)                                                                          \
   case N:                                                                                \
     if (mode == FR_FWD) hipLaunchKernelGGL((k_frame<N, FR_FWD>), grid, block, 0, s, a);  \
