@@ -72,7 +72,21 @@ typedef struct rtm_params {
                             j = m*snap_every, 2 <= j, j+1 <= nt-3, in strip_format with a
                             per-snapshot power-of-two scale; the backward pass replaces
                             the reconstructed u(k) by its snapshot */
+  int stack_nx;          /* columns of the stacked image (0 = nx - 2*n2); a ragged batch
+                            (rtm_set_shot_geometry) stacks each shot at its own offset */
 } rtm_params;
+
+/* One shot of a ragged batch (uneven apertures; not in the reference, whose
+ * runs share one grid): the shot's grid is nx columns wide (nx <= the
+ * context's nx, same nz and n2), its receivers are s_l, s_l+ds, .., s_r on
+ * row s_z of the context (n <= the context's n), and its interior column 0
+ * lands in column x_img of the stacked image. */
+typedef struct rtm_shot_geom {
+  int nx;
+  int r_u, r_x;          /* source, 0-based padded, inside the shot's grid */
+  int s_l, s_r, n;       /* receivers, 0-based padded */
+  int x_img;
+} rtm_shot_geom;
 
 typedef struct rtm_ctx rtm_ctx;
 
@@ -94,6 +108,16 @@ int rtm_set_model(rtm_ctx* ctx, const float* v, const float* r1, const float* w,
 int rtm_set_wavelet(rtm_ctx* ctx, const float* wav);
 /* Sources of the batch: shot i injects at (r_u[i], r_x[i]) (kernel.cu:794-795). */
 int rtm_set_shots(rtm_ctx* ctx, int nshot, const int* r_u, const int* r_x);
+/* A ragged batch: shot i has its own grid width, receivers, source and image
+ * offset.  No counterpart in the reference (a drop-in caller keeps
+ * rtm_set_shots).  All shots run in the same launches. */
+int rtm_set_shot_geometry(rtm_ctx* ctx, int nshot, const rtm_shot_geom* geom);
+/* Model window of one shot of a ragged batch: v, r1 [nz*geom.nx] as velocity()
+ * builds them for that window.  Call after rtm_set_model (which fixes the
+ * coefficient table, w, vmin and dv) and rtm_set_shot_geometry; once any shot
+ * has a window, every shot of the batch must have one.  rtm_set_model returns
+ * the context to one model for all shots. */
+int rtm_set_shot_model(rtm_ctx* ctx, int shot, const float* v, const float* r1);
 /* Observed gathers of the batch, seis[shot][n][nt] as read at kernel.cu:827-856.
  * Uploaded once per batch (the reference copies one time slice per step,
  * kernel.cu:905-909). */

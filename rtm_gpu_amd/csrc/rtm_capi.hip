@@ -50,14 +50,22 @@ struct rtm_ctx {
   float* Q[3] = {nullptr, nullptr, nullptr};
   float *SU = nullptr, *SQ = nullptr, *I1 = nullptr, *I2 = nullptr;
   float *d_data = nullptr, *d_rec = nullptr;
-  int2* d_src = nullptr;
-  int4* d_tiles = nullptr;
-  int ntiles = 0;
+  ShotGeo* d_geo = nullptr;    // per shot of the batch
+  std::vector<ShotGeo> geo;
+  int gw = 0;                  // columns of the stacked image
+  int4* d_tiles = nullptr;     // frame (tile, shot) records of the batch
+  int ntiles = 0, tile_cap = 0;
+  // per-shot model windows of a ragged batch (allocated on first use)
+  float *d_mv = nullptr, *d_mr1 = nullptr, *d_mpm = nullptr;
+  unsigned* d_mko = nullptr;
+  std::vector<char> shot_model;
+  float tab_vmin = 0.f, tab_dv = 1.f;  // bin parameters of the current table
+  int tab_nrow = 1, tab_cs = 4;
+  bool tab_ls = true;
   float *d_up = nullptr, *d_down = nullptr, *d_s1 = nullptr, *d_s2 = nullptr;
   unsigned* d_imax = nullptr;
   // host-side state
   std::vector<float> wav;
-  std::vector<int2> src;
   int nshot = 0;
   int f0 = 0, f1 = 1, f2 = 2, q0 = 0, q1 = 1, q2 = 2;
   int kf = -1, kb = -2;  // next forward / backward step; -1/-2 = not begun
@@ -174,7 +182,7 @@ void fill_int_args(rtm_ctx* c, IntArgs& a) {
   a.g = c->g;
   a.m = c->m;
   a.r = c->r;
-  a.src = c->d_src;
+  a.geo = c->d_geo;
   a.nt = c->p.nt;
   a.nxt = c->nxt;
   a.zs = c->zs;
@@ -188,7 +196,7 @@ void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
   a.m = c->m;
   a.r = c->r;
   a.tiles = c->d_tiles;
-  a.src = c->d_src;
+  a.geo = c->d_geo;
   a.nt = c->p.nt;
   a.f64comb = c->p.ilste != 0;
   a.data = c->d_data;
@@ -199,7 +207,7 @@ void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
 int forward_step(rtm_ctx* c, int k) {
   const float wv = (k < c->p.nt2) ? c->wav[k - 1] : 0.0f;  // kernel.cu:812-813
   const bool quant = c->save_strips && c->st.fmt != RTM_STRIP_F32;
-  if (quant && k >= 3) HIPCHK(c, launch_strip_finish(c->g, c->st, k - 1, c->nshot, c->stream));
+  if (quant && k >= 3) HIPCHK(c, launch_strip_finish(c->g, c->st, c->d_geo, k - 1, c->nshot, c->stream));
   {
     IntArgs a;
     fill_int_args(c, a);
@@ -218,11 +226,11 @@ int forward_step(rtm_ctx* c, int k) {
     Timed t(c, RTM_K_FWD_FRAME);
     HIPCHK(c, launch_frame(FR_FWD, c->nfd, a, c->nshot, c->ntiles, c->stream));
   }
-  if (quant && k == c->p.nt - 1) HIPCHK(c, launch_strip_finish(c->g, c->st, k, c->nshot, c->stream));
+  if (quant && k == c->p.nt - 1) HIPCHK(c, launch_strip_finish(c->g, c->st, c->d_geo, k, c->nshot, c->stream));
   if (c->save_strips && c->snap_slot[k] >= 0)  // checkpoint snapshot of u(k)
-    HIPCHK(c, launch_snap_save(c->g, c->F[c->f2], c->sn, c->snap_slot[k], c->nshot, c->stream));
+    HIPCHK(c, launch_snap_save(c->g, c->F[c->f2], c->sn, c->d_geo, c->snap_slot[k], c->nshot, c->stream));
   if (c->p.record)  // receiver recording of u(k)
-    HIPCHK(c, launch_record(c->g, c->r, c->F[c->f2], c->d_rec, k, c->p.nt, c->nshot, c->stream));
+    HIPCHK(c, launch_record(c->g, c->r, c->d_geo, c->F[c->f2], c->d_rec, k, c->p.nt, c->nshot, c->stream));
   int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t;  // Deliver (kernel.cu:820)
   return RTM_OK;
 }
@@ -239,10 +247,10 @@ int backward_step(rtm_ctx* c, int k) {
   }
   {  // strips of slot k into the rings of u(k): BKEqual of step k-1 (kernel.cu:892)
     Timed t(c, RTM_K_STRIP_RESTORE);
-    HIPCHK(c, launch_strip_restore(c->g, c->F[c->f2], c->st, k, c->nshot, c->stream));
+    HIPCHK(c, launch_strip_restore(c->g, c->F[c->f2], c->st, c->d_geo, k, c->nshot, c->stream));
   }
   if (c->snap_slot[k] >= 0)  // checkpoint: the reconstructed u(k) is replaced by its snapshot
-    HIPCHK(c, launch_snap_restore(c->g, c->F[c->f2], c->sn, c->snap_slot[k], c->nshot, c->stream));
+    HIPCHK(c, launch_snap_restore(c->g, c->F[c->f2], c->sn, c->d_geo, c->snap_slot[k], c->nshot, c->stream));
   { int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t; }  // Deliver_EFF (kernel.cu:903)
   {  // BKAdd (kernel.cu:913) + Rel_* (kernel.cu:923-930)
     IntArgs a;
@@ -265,17 +273,84 @@ int backward_step(rtm_ctx* c, int k) {
   return RTM_OK;
 }
 
-void build_tiles(rtm_ctx* c, std::vector<int4>& t) {
-  const Grid& g = c->g;
-  // tile shapes must match k_frame: 32 x 64 (top/bottom), 64 x 32 (left/right)
+// Frame tiles of a grid nx columns wide, tagged with the shot.  Shapes must
+// match k_frame: 32 x 64 (top/bottom), 64 x 32 (left/right).
+void build_tiles(const Grid& g, int nx, int shot, std::vector<int4>& t) {
   for (int z0 = 0; z0 < g.n2; z0 += 32)
-    for (int x0 = 0; x0 < g.nx; x0 += 64) t.push_back(make_int4(z0, x0, 0, 0));
+    for (int x0 = 0; x0 < nx; x0 += 64) t.push_back(make_int4(z0, x0, 0, shot));
   for (int z0 = g.nz - g.n2; z0 < g.nz; z0 += 32)
-    for (int x0 = 0; x0 < g.nx; x0 += 64) t.push_back(make_int4(z0, x0, 1, 0));
+    for (int x0 = 0; x0 < nx; x0 += 64) t.push_back(make_int4(z0, x0, 1, shot));
   for (int z0 = g.n2; z0 < g.nz - g.n2; z0 += 64) {
-    for (int x0 = 0; x0 < g.n2; x0 += 32) t.push_back(make_int4(z0, x0, 2, 0));
-    for (int x0 = g.nx - g.n2; x0 < g.nx; x0 += 32) t.push_back(make_int4(z0, x0, 3, 0));
+    for (int x0 = 0; x0 < g.n2; x0 += 32) t.push_back(make_int4(z0, x0, 2, shot));
+    for (int x0 = nx - g.n2; x0 < nx; x0 += 32) t.push_back(make_int4(z0, x0, 3, shot));
   }
+}
+
+int choose_zs(rtm_ctx* c, int nshot);
+void drop_graph(rtm_ctx* c);
+
+// Upload the batch geometry and its frame records: tile-major with the shot
+// fastest, so that one XCD holds many shots of the same tiles at a time.
+int upload_geometry(rtm_ctx* c) {
+  const int ns = (int)c->geo.size();
+  std::vector<std::vector<int4>> per(ns);
+  size_t most = 0;
+  for (int s = 0; s < ns; ++s) {
+    build_tiles(c->g, c->geo[s].nx, s, per[s]);
+    most = std::max(most, per[s].size());
+  }
+  std::vector<int4> rec;
+  for (size_t t = 0; t < most; ++t)
+    for (int s = 0; s < ns; ++s)
+      if (t < per[s].size()) rec.push_back(per[s][t]);
+  if ((int)rec.size() > c->tile_cap) return fail(c, RTM_E_ARG, "frame records exceed capacity");
+  HIPCHK(c, hipMemcpyAsync(c->d_geo, c->geo.data(), ns * sizeof(ShotGeo), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_tiles, rec.data(), rec.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ntiles = (int)rec.size();
+  c->nshot = ns;
+  choose_zs(c, ns);
+  drop_graph(c);
+  c->shots_ok = true;
+  c->data_ok = false;
+  c->kf = -1;
+  c->kb = -2;
+  return RTM_OK;
+}
+
+// Pad a model window (v, r1 of nz x nx) into the padded layout and derive the
+// per-cell operands: p = v*v*tao2*h2 (kernel.cu:73) and the byte offset of the
+// velocity bin's coefficient row (kernel.cu:55-56: FP32 quotient, double +
+// 0.5, truncation, clamped to the table).  Guard columns repeat the edge.
+int upload_model_window(rtm_ctx* c, int nx, const float* v, const float* r1, float* dv_, float* dr_,
+                        float* dp_, unsigned* dk_) {
+  const Grid& g = c->g;
+  std::vector<float> pv((size_t)g.fstride, 0.f), pr((size_t)g.fstride, 0.f), pp((size_t)g.fstride);
+  std::vector<unsigned> pk((size_t)g.fstride);
+  for (int z = 0; z < g.nz; ++z)
+    for (int x = 0; x < g.pitch; ++x) {
+      const int xs = std::min(std::max(x - g.x0, 0), nx - 1);
+      pv[(size_t)z * g.pitch + x] = v[(size_t)z * nx + xs];
+      pr[(size_t)z * g.pitch + x] = r1[(size_t)z * nx + xs];
+    }
+  for (size_t i = 0; i < pv.size(); ++i) {
+    const float vv = pv[i];
+    pp[i] = vv * vv * c->p.tao2 * c->p.h2;
+    int kb = 0;
+    if (c->tab_ls) {
+      const float qf = (vv - c->tab_vmin) / c->tab_dv;
+      kb = (int)((double)qf + 0.5);
+      kb = std::min(std::max(kb, 0), c->tab_nrow - 1);
+    }
+    pk[i] = (unsigned)kb * (unsigned)c->tab_cs * 4u;
+  }
+  const size_t b = (size_t)g.fstride * sizeof(float);
+  HIPCHK(c, hipMemcpyAsync(dv_, pv.data(), b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dr_, pr.data(), b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dp_, pp.data(), b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(dk_, pk.data(), b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RTM_OK;
 }
 
 int choose_zs(rtm_ctx* c, int nshot) {
@@ -323,6 +398,7 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
     rc = fail(c, RTM_E_ARG, "receiver line outside grid");
   else if (p->strip_format < 0 || p->strip_format > 2) rc = fail(c, RTM_E_ARG, "strip_format");
   else if (p->snap_every < 0) rc = fail(c, RTM_E_ARG, "snap_every=%d < 0", p->snap_every);
+  else if (p->stack_nx < 0) rc = fail(c, RTM_E_ARG, "stack_nx=%d < 0", p->stack_nx);
   if (rc != RTM_OK) {
     int code = rc;
     delete c;
@@ -363,7 +439,7 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_pm, (size_t)g.fstride);
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_ko, (size_t)g.fstride);
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_w, (size_t)g.n2 + 1);
-  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_src, (size_t)p->max_shots);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_geo, (size_t)p->max_shots);
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_data, (size_t)p->max_shots * p->nt * p->n);
   if (rc == RTM_OK && p->record) rc = dalloc_t(c, &c->d_rec, (size_t)p->max_shots * p->nt * p->n);
   Strips& st = c->st;
@@ -397,22 +473,24 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
     }
   }
   const size_t nI = (size_t)g.mnz * g.mnx;
+  c->gw = p->stack_nx > 0 ? p->stack_nx : g.mnx;
+  const size_t nS = (size_t)g.mnz * c->gw;
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_up, (size_t)p->max_shots * nI);
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_down, (size_t)p->max_shots * nI);
-  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_s1, nI);
-  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_s2, nI);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_s1, std::max(nS, nI));
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_s2, std::max(nS, nI));
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_imax, (size_t)p->max_shots);
-  std::vector<int4> tiles;
-  build_tiles(c, tiles);
-  c->ntiles = (int)tiles.size();
-  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_tiles, tiles.size());
+  {
+    std::vector<int4> tiles;  // the widest grid has the most frame tiles
+    build_tiles(g, g.nx, 0, tiles);
+    c->tile_cap = (int)tiles.size() * p->max_shots;
+  }
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_tiles, (size_t)c->tile_cap);
   if (rc != RTM_OK) {
     int code = rc;
     rtm_destroy(c);
     return code;
   }
-  HIPCHK(c, hipMemcpyAsync(c->d_tiles, tiles.data(), tiles.size() * sizeof(int4),
-                           hipMemcpyHostToDevice, c->stream));
   // Zero everything once so guard columns hold finite values.
   for (int i = 0; i < 3; ++i) {
     HIPCHK(c, launch_fill(c->F[i], (long long)fb, 0.f, c->stream));
@@ -422,8 +500,8 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
   HIPCHK(c, launch_fill(c->SQ, (long long)fb, 0.f, c->stream));
   HIPCHK(c, launch_fill(c->I1, (long long)fb, 0.f, c->stream));
   HIPCHK(c, launch_fill(c->I2, (long long)fb, 0.f, c->stream));
-  HIPCHK(c, launch_fill(c->d_s1, (long long)nI, 0.f, c->stream));
-  HIPCHK(c, launch_fill(c->d_s2, (long long)nI, 0.f, c->stream));
+  HIPCHK(c, launch_fill(c->d_s1, (long long)std::max(nS, nI), 0.f, c->stream));
+  HIPCHK(c, launch_fill(c->d_s2, (long long)std::max(nS, nI), 0.f, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *out = c;
   return RTM_OK;
@@ -482,43 +560,17 @@ int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, c
     c->ctab_n = tab.size();
   }
   HIPCHK(c, hipMemcpyAsync(c->d_ctab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  // model arrays into the padded layout; guard columns repeat the edge value
-  std::vector<float> pv((size_t)g.fstride, 0.f), pr((size_t)g.fstride, 0.f);
-  for (int z = 0; z < g.nz; ++z) {
-    float* dv_ = &pv[(size_t)z * g.pitch];
-    float* dr_ = &pr[(size_t)z * g.pitch];
-    for (int x = 0; x < g.pitch; ++x) {
-      int xs = std::min(std::max(x - g.x0, 0), g.nx - 1);
-      dv_[x] = v[(size_t)z * g.nx + xs];
-      dr_[x] = r1[(size_t)z * g.nx + xs];
-    }
-  }
-  HIPCHK(c, hipMemcpyAsync(c->d_v, pv.data(), pv.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->d_r1, pr.data(), pr.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  // Per-cell operands of the interior kernel, from the same FP32 expressions:
-  // p = v*v*tao2*h2 (kernel.cu:73) and the velocity bin of kernel.cu:55-56
-  // (FP32 quotient, double + 0.5, truncation), clamped to the table.
-  std::vector<float> pp((size_t)g.fstride);
-  std::vector<unsigned> pk((size_t)g.fstride);
-  for (size_t i = 0; i < pv.size(); ++i) {
-    const float vv = pv[i];
-    pp[i] = vv * vv * c->p.tao2 * c->p.h2;
-    int kb = 0;
-    if (ls) {
-      const float qf = (vv - vmin) / dv;
-      kb = (int)((double)qf + 0.5);
-      kb = std::min(std::max(kb, 0), nrow - 1);
-    }
-    pk[i] = (unsigned)kb * (unsigned)CS * 4u;
-  }
-  HIPCHK(c, hipMemcpyAsync(c->d_pm, pp.data(), pp.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->d_ko, pk.data(), pk.size() * sizeof(unsigned), hipMemcpyHostToDevice, c->stream));
+  c->tab_vmin = vmin; c->tab_dv = dv; c->tab_nrow = nrow; c->tab_cs = CS; c->tab_ls = ls;
+  rc = upload_model_window(c, g.nx, v, r1, c->d_v, c->d_r1, c->d_pm, c->d_ko);
+  if (rc != RTM_OK) return rc;
   HIPCHK(c, hipMemcpyAsync(c->d_w, w, (g.n2 + 1) * sizeof(float), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   Model& m = c->m;
+  m.mstride = 0;  // one model for every shot until rtm_set_shot_model
   m.v = c->d_v; m.r1 = c->d_r1; m.ctab = c->d_ctab; m.w = c->d_w;
   m.pm = c->d_pm; m.ko = c->d_ko;
   m.cs = CS; m.nrow = nrow;
+  c->shot_model.assign(c->p.max_shots, 0);
   m.hzx2_1 = c->p.hzx2_1; m.tao2 = c->p.tao2; m.h2 = c->p.h2;
   m.taoh = c->p.taoh; m.taoh2 = c->p.taoh2;
   c->nfd = nfd;
@@ -538,21 +590,62 @@ int rtm_set_wavelet(rtm_ctx* c, const float* wav) {
 int rtm_set_shots(rtm_ctx* c, int nshot, const int* r_u, const int* r_x) {
   if (!c || !r_u || !r_x) return RTM_E_ARG;
   if (nshot < 1 || nshot > c->p.max_shots) return fail(c, RTM_E_ARG, "nshot=%d outside 1..%d", nshot, c->p.max_shots);
-  c->src.resize(nshot);
+  const Grid& g = c->g;
+  std::vector<ShotGeo> geo(nshot);
   for (int i = 0; i < nshot; ++i) {
-    if (r_u[i] < 0 || r_u[i] >= c->g.nz || r_x[i] < 0 || r_x[i] >= c->g.nx)
+    if (r_u[i] < 0 || r_u[i] >= g.nz || r_x[i] < 0 || r_x[i] >= g.nx)
       return fail(c, RTM_E_ARG, "shot %d source (%d,%d) outside grid", i, r_u[i], r_x[i]);
-    c->src[i] = make_int2(r_u[i], r_x[i]);
+    geo[i] = ShotGeo{g.nx, g.mnx, c->p.s_l, c->p.s_r, c->p.n, r_u[i], r_x[i], 0};
   }
-  HIPCHK(c, hipMemcpyAsync(c->d_src, c->src.data(), nshot * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->nshot = nshot;
-  choose_zs(c, nshot);
+  if (c->m.mstride != 0) return fail(c, RTM_E_STATE, "per-shot model windows need rtm_set_shot_geometry");
+  c->geo = geo;
+  return upload_geometry(c);
+}
+
+int rtm_set_shot_geometry(rtm_ctx* c, int nshot, const rtm_shot_geom* sg) {
+  if (!c || !sg) return RTM_E_ARG;
+  if (nshot < 1 || nshot > c->p.max_shots) return fail(c, RTM_E_ARG, "nshot=%d outside 1..%d", nshot, c->p.max_shots);
+  const Grid& g = c->g;
+  std::vector<ShotGeo> geo(nshot);
+  for (int i = 0; i < nshot; ++i) {
+    const rtm_shot_geom& q = sg[i];
+    const int mnx = q.nx - 2 * g.n2;
+    if (q.nx > g.nx || mnx < 2)
+      return fail(c, RTM_E_ARG, "shot %d: width %d outside %d..%d", i, q.nx, 2 * g.n2 + 2, g.nx);
+    if (q.r_u < 0 || q.r_u >= g.nz || q.r_x < 0 || q.r_x >= q.nx)
+      return fail(c, RTM_E_ARG, "shot %d source (%d,%d) outside its grid", i, q.r_u, q.r_x);
+    if (q.n < 1 || q.n > c->p.n || q.s_l < 0 || q.s_r != q.s_l + (q.n - 1) * c->p.ds || q.s_r >= q.nx)
+      return fail(c, RTM_E_ARG, "shot %d receivers s_l=%d s_r=%d n=%d", i, q.s_l, q.s_r, q.n);
+    if (q.x_img < 0 || q.x_img + mnx > c->gw)
+      return fail(c, RTM_E_ARG, "shot %d image columns %d..%d outside 0..%d", i, q.x_img, q.x_img + mnx, c->gw);
+    geo[i] = ShotGeo{q.nx, mnx, q.s_l, q.s_r, q.n, q.r_u, q.r_x, q.x_img};
+  }
+  c->geo = geo;
+  return upload_geometry(c);
+}
+
+int rtm_set_shot_model(rtm_ctx* c, int shot, const float* v, const float* r1) {
+  if (!c || !v || !r1) return RTM_E_ARG;
+  if (!c->model_ok) return fail(c, RTM_E_STATE, "rtm_set_model first (coefficient table)");
+  if (!c->shots_ok || shot < 0 || shot >= c->nshot) return fail(c, RTM_E_ARG, "shot %d of the batch", shot);
+  const Grid& g = c->g;
+  if (!c->d_mv) {
+    const size_t n = (size_t)c->p.max_shots * g.fstride;
+    int rc = dalloc_t(c, &c->d_mv, n);
+    if (rc == RTM_OK) rc = dalloc_t(c, &c->d_mr1, n);
+    if (rc == RTM_OK) rc = dalloc_t(c, &c->d_mpm, n);
+    if (rc == RTM_OK) rc = dalloc_t(c, &c->d_mko, n);
+    if (rc != RTM_OK) return rc;
+  }
+  const size_t o = (size_t)shot * g.fstride;
+  int rc = upload_model_window(c, c->geo[shot].nx, v, r1, c->d_mv + o, c->d_mr1 + o, c->d_mpm + o,
+                               c->d_mko + o);
+  if (rc != RTM_OK) return rc;
+  Model& m = c->m;
+  m.mstride = g.fstride;
+  m.v = c->d_mv; m.r1 = c->d_mr1; m.pm = c->d_mpm; m.ko = c->d_mko;
+  c->shot_model[shot] = 1;
   drop_graph(c);
-  c->shots_ok = true;
-  c->data_ok = false;
-  c->kf = -1;
-  c->kb = -2;
   return RTM_OK;
 }
 
@@ -586,23 +679,26 @@ int rtm_gathers_from_records(rtm_ctx* c) {
 int rtm_forward_begin(rtm_ctx* c, int save_strips) {
   if (!c) return RTM_E_ARG;
   if (!c->model_ok || !c->wav_ok || !c->shots_ok) return fail(c, RTM_E_STATE, "model, wavelet and shots must be set");
+  if (c->m.mstride != 0)
+    for (int s = 0; s < c->nshot; ++s)
+      if (!c->shot_model[s]) return fail(c, RTM_E_STATE, "shot %d has no model window", s);
   c->f0 = 0; c->f1 = 1; c->f2 = 2;
   c->save_strips = save_strips != 0;
   const float spike = (float)((double)c->wav[0] / 2.0);  // f(0.0,f0)/2.0 (kernel.cu:803)
-  HIPCHK(c, launch_fwd_begin(c->g, c->F[0], c->F[1], c->d_src, spike, c->nshot, c->stream));
+  HIPCHK(c, launch_fwd_begin(c->g, c->F[0], c->F[1], c->d_geo, spike, c->nshot, c->stream));
   if (c->save_strips) {  // Equal (kernel.cu:808)
     if (c->st.fmt != RTM_STRIP_F32)
       HIPCHK(c, launch_fill(reinterpret_cast<float*>(c->st.amax), (long long)c->nshot * c->st.nslot, 0.f, c->stream));
-    HIPCHK(c, launch_strip_save(c->g, c->F[0], c->st, 0, c->nshot, c->stream));
-    HIPCHK(c, launch_strip_save(c->g, c->F[1], c->st, 1, c->nshot, c->stream));
+    HIPCHK(c, launch_strip_save(c->g, c->F[0], c->st, c->d_geo, 0, c->nshot, c->stream));
+    HIPCHK(c, launch_strip_save(c->g, c->F[1], c->st, c->d_geo, 1, c->nshot, c->stream));
     if (c->st.fmt != RTM_STRIP_F32) {
-      HIPCHK(c, launch_strip_finish(c->g, c->st, 0, c->nshot, c->stream));
-      HIPCHK(c, launch_strip_finish(c->g, c->st, 1, c->nshot, c->stream));
+      HIPCHK(c, launch_strip_finish(c->g, c->st, c->d_geo, 0, c->nshot, c->stream));
+      HIPCHK(c, launch_strip_finish(c->g, c->st, c->d_geo, 1, c->nshot, c->stream));
     }
   }
   if (c->p.record) {
-    HIPCHK(c, launch_record(c->g, c->r, c->F[0], c->d_rec, 0, c->p.nt, c->nshot, c->stream));
-    HIPCHK(c, launch_record(c->g, c->r, c->F[1], c->d_rec, 1, c->p.nt, c->nshot, c->stream));
+    HIPCHK(c, launch_record(c->g, c->r, c->d_geo, c->F[0], c->d_rec, 0, c->p.nt, c->nshot, c->stream));
+    HIPCHK(c, launch_record(c->g, c->r, c->d_geo, c->F[1], c->d_rec, 1, c->p.nt, c->nshot, c->stream));
   }
   c->kf = 2;
   c->kb = -2;
@@ -629,9 +725,9 @@ int rtm_backward_begin(rtm_ctx* c) {
   c->q0 = 0; c->q1 = 1; c->q2 = 2;
   const float spike = (float)((double)c->wav[0] / 2.0);
   HIPCHK(c, launch_bwd_begin(c->g, c->F[c->f0], c->F[c->f1], c->Q[0], c->Q[1], c->SU, c->SQ,
-                             c->I1, c->I2, c->d_src, spike, c->p.icompen, c->nshot, c->stream));
+                             c->I1, c->I2, c->d_geo, spike, c->p.icompen, c->nshot, c->stream));
   // BKEqual of the first backward step (kernel.cu:892, k = NT-3): slot NT-2.
-  HIPCHK(c, launch_strip_restore(c->g, c->F[c->f1], c->st, c->p.nt - 2, c->nshot, c->stream));
+  HIPCHK(c, launch_strip_restore(c->g, c->F[c->f1], c->st, c->d_geo, c->p.nt - 2, c->nshot, c->stream));
   c->kb = c->p.nt - 3;
   return RTM_OK;
 }
@@ -722,14 +818,15 @@ int rtm_read_field(rtm_ctx* c, int shot, int which, float* out) {
     default: return fail(c, RTM_E_ARG, "field %d", which);
   }
   const Grid& g = c->g;
+  const int nx = c->geo[shot].nx;  // the shot's grid fills the first nx columns
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy2D(out, g.nx * sizeof(float), base + (size_t)shot * g.fstride + g.x0,
-                        g.pitch * sizeof(float), g.nx * sizeof(float), g.nz, hipMemcpyDeviceToHost));
-  if (which >= RTM_FIELD_I1) {  // images are defined on the interior only
-    for (int z = 0; z < g.nz; ++z)
-      for (int x = 0; x < g.nx; ++x)
-        if (z < g.n2 || z >= g.nz - g.n2 || x < g.n2 || x >= g.nx - g.n2) out[(size_t)z * g.nx + x] = 0.f;
-  }
+                        g.pitch * sizeof(float), nx * sizeof(float), g.nz, hipMemcpyDeviceToHost));
+  for (int z = 0; z < g.nz; ++z)
+    for (int x = 0; x < g.nx; ++x)
+      if (x >= nx || (which >= RTM_FIELD_I1 &&  // images are defined on the interior only
+                      (z < g.n2 || z >= g.nz - g.n2 || x < g.n2 || x >= nx - g.n2)))
+        out[(size_t)z * g.nx + x] = 0.f;
   return RTM_OK;
 }
 
@@ -780,12 +877,13 @@ int rtm_image_post(rtm_ctx* c, float vmax, float whitecoe, float* up, float* dow
   if (!c) return RTM_E_ARG;
   if (c->kb != -1) return fail(c, RTM_E_STATE, "backward pass incomplete");
   const Grid& g = c->g;
-  const int nI = g.mnz * g.mnx;
+  const int nI = g.mnz * g.mnx;  // per shot: [mnz][mnx of the widest shot]
   HIPCHK(c, launch_fill(reinterpret_cast<float*>(c->d_imax), c->nshot, 0.f, c->stream));
-  HIPCHK(c, launch_maxabs(g, c->I2, c->d_imax, c->nshot, c->stream));
-  HIPCHK(c, launch_post(g, c->d_v, c->I1, c->I2, c->d_imax, vmax, whitecoe, c->d_up, c->d_down,
-                        c->nshot, c->stream));
-  HIPCHK(c, launch_stack_add(nI, c->d_up, c->d_down, c->d_s1, c->d_s2, c->nshot, c->stream));
+  HIPCHK(c, launch_maxabs(g, c->d_geo, c->I2, c->d_imax, c->nshot, c->stream));
+  HIPCHK(c, launch_post(g, c->d_geo, c->m.v, c->m.mstride, c->I1, c->I2, c->d_imax, vmax, whitecoe,
+                        c->d_up, c->d_down, c->nshot, c->stream));
+  HIPCHK(c, launch_stack_add(g, c->d_geo, c->gw, c->d_up, c->d_down, c->d_s1, c->d_s2, c->nshot,
+                             c->stream));
   if (up) HIPCHK(c, hipMemcpyAsync(up, c->d_up, sizeof(float) * nI * c->nshot, hipMemcpyDeviceToHost, c->stream));
   if (down) HIPCHK(c, hipMemcpyAsync(down, c->d_down, sizeof(float) * nI * c->nshot, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -794,7 +892,7 @@ int rtm_image_post(rtm_ctx* c, float vmax, float whitecoe, float* up, float* dow
 
 int rtm_stack_finish(rtm_ctx* c, int nrec, int inorm, float* img, float* illum) {
   if (!c || !img || nrec < 1) return RTM_E_ARG;
-  const int nI = c->g.mnz * c->g.mnx;
+  const int nI = c->g.mnz * c->gw;  // the stacked image
   // d_up/d_down of shot 0 are reused as the result buffers.
   HIPCHK(c, launch_stack_finish(nI, c->d_s1, c->d_s2, nrec, inorm, c->d_up, c->d_down, c->stream));
   HIPCHK(c, hipMemcpyAsync(img, c->d_up, sizeof(float) * nI, hipMemcpyDeviceToHost, c->stream));

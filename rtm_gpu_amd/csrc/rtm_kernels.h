@@ -11,7 +11,7 @@ namespace rtm {
 // first interior column n2 is 16-byte aligned, and the row has >= 4 guard
 // columns on each side so every float4 row access stays inside its row.
 struct Grid {
-  int nz, nx, pitch, x0, n2, mnz, mnx;
+  int nz, nx, pitch, x0, n2, mnz, mnx;  // nx, mnx: the widest shot of the batch
   int nfd;          // nfdmax of the model (strip depth)
   long long fstride;
 };
@@ -19,6 +19,7 @@ struct Grid {
 // Model operands.  ctab rows are [c0' as double | c_1 .. c_NFD | zero pad],
 // c0' = (1.0 + hzx2_1) * c_0 in double (the promotion of kernel.cu:58).
 struct Model {
+  long long mstride;   // 0: one model for all shots; fstride: one model window per shot
   const float* v;
   const float* r1;
   const float* pm;     // v*v*tao2*h2 per cell (FP32, left to right), padded layout
@@ -30,7 +31,18 @@ struct Model {
 };
 
 struct Rcv {
-  int s_l, s_r, s_z, ds, n;
+  int s_l, s_r, s_z, ds, n;  // n: receivers per shot of the data/record layout (largest shot)
+};
+
+// Geometry of one shot of a batch.  A uniform batch repeats the context's
+// grid and receiver line; a ragged batch gives each shot its own aperture
+// window: a narrower grid (nx <= Grid::nx, same nz) with its own receivers,
+// and the column of the global image its interior starts at.
+struct ShotGeo {
+  int nx, mnx;         // padded / interior width
+  int s_l, s_r, n;     // receivers on row s_z, spacing ds (common)
+  int r_u, r_x;        // source
+  int x_img;           // image column of interior column 0
 };
 
 // Strip storage: fp32, or fp16/bf16 with a per-(shot, slot) scale.
@@ -61,7 +73,7 @@ struct IntArgs {
   const float* u1;
   const float* u0;
   float* u2;
-  const int2* src;     // per shot (r_u, r_x)
+  const ShotGeo* geo;  // per shot
   float wav;           // source amplitude this step (0 past the gate)
   int k, nt;
   int nxt, zs;         // interior x tiles, rows per wave (requested)
@@ -79,8 +91,8 @@ struct FrameArgs {
   const float* u1;
   const float* u0;
   float* u2;
-  const int4* tiles;   // (z0, x0, band, 0)
-  const int2* src;
+  const int4* tiles;   // (z0, x0, band, shot)
+  const ShotGeo* geo;
   float wav;
   int k, nt;
   int f64comb;
@@ -96,31 +108,38 @@ struct FrameArgs {
 // multiple of it.
 int interior_queue_rows(int epi, int nfd);
 hipError_t launch_interior(int epi, int nfd_inst, IntArgs a, int nshot, hipStream_t s);
-hipError_t launch_frame(int mode, int nfd_inst, FrameArgs a, int nshot, int ntiles,
+hipError_t launch_frame(int mode, int nfd_inst, FrameArgs a, int nshot, int nrecords,
                         hipStream_t s);
 hipError_t launch_fill(float* p, long long n, float val, hipStream_t s);
-hipError_t launch_fwd_begin(const Grid& g, float* f0, float* f1, const int2* src, float spike,
+// Per-shot kernels take the batch's geometry array; launches cover shots
+// 0..nshot-1 (blockIdx.y).
+hipError_t launch_fwd_begin(const Grid& g, float* f0, float* f1, const ShotGeo* geo, float spike,
                             int nshot, hipStream_t s);
-hipError_t launch_strip_save(const Grid& g, const float* f, Strips st, int k, int nshot,
-                             hipStream_t s);
-hipError_t launch_strip_restore(const Grid& g, float* f, Strips st, int k, int nshot,
-                                hipStream_t s);
-hipError_t launch_strip_finish(const Grid& g, Strips st, int k, int nshot, hipStream_t s);
-hipError_t launch_snap_save(const Grid& g, const float* f, Snaps sn, int slot, int nshot,
-                            hipStream_t s);
-hipError_t launch_snap_restore(const Grid& g, float* f, Snaps sn, int slot, int nshot,
+hipError_t launch_strip_save(const Grid& g, const float* f, Strips st, const ShotGeo* geo, int k,
+                             int nshot, hipStream_t s);
+hipError_t launch_strip_restore(const Grid& g, float* f, Strips st, const ShotGeo* geo, int k,
+                                int nshot, hipStream_t s);
+hipError_t launch_strip_finish(const Grid& g, Strips st, const ShotGeo* geo, int k, int nshot,
                                hipStream_t s);
-hipError_t launch_record(const Grid& g, const Rcv& r, const float* f, float* rec, int k, int nt,
-                         int nshot, hipStream_t s);
+hipError_t launch_snap_save(const Grid& g, const float* f, Snaps sn, const ShotGeo* geo, int slot,
+                            int nshot, hipStream_t s);
+hipError_t launch_snap_restore(const Grid& g, float* f, Snaps sn, const ShotGeo* geo, int slot,
+                               int nshot, hipStream_t s);
+hipError_t launch_record(const Grid& g, const Rcv& r, const ShotGeo* geo, const float* f,
+                         float* rec, int k, int nt, int nshot, hipStream_t s);
 hipError_t launch_bwd_begin(const Grid& g, const float* s0, const float* s1, float* q0,
                             float* q1, float* su, float* sq, float* i1, float* i2,
-                            const int2* src, float spike, int icompen, int nshot, hipStream_t s);
-hipError_t launch_maxabs(const Grid& g, const float* i2, unsigned* out, int nshot, hipStream_t s);
-hipError_t launch_post(const Grid& g, const float* v, const float* i1, const float* i2,
-                       const unsigned* amax, float vmax, float whitecoe, float* up, float* down,
-                       int nshot, hipStream_t s);
-hipError_t launch_stack_add(int nI, const float* up, const float* down, float* s1, float* s2,
-                            int nshot, hipStream_t s);
+                            const ShotGeo* geo, float spike, int icompen, int nshot, hipStream_t s);
+hipError_t launch_maxabs(const Grid& g, const ShotGeo* geo, const float* i2, unsigned* out,
+                         int nshot, hipStream_t s);
+// up/down: [shot][mnz][g.mnx] (each shot fills its first mnx columns)
+hipError_t launch_post(const Grid& g, const ShotGeo* geo, const float* v, long long mstride,
+                       const float* i1, const float* i2, const unsigned* amax, float vmax,
+                       float whitecoe, float* up, float* down, int nshot, hipStream_t s);
+// s1/s2: the stacked image, mnz x gw; shot s adds into columns x_img..x_img+mnx-1,
+// in shot order per cell.
+hipError_t launch_stack_add(const Grid& g, const ShotGeo* geo, int gw, const float* up,
+                            const float* down, float* s1, float* s2, int nshot, hipStream_t s);
 hipError_t launch_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
                                float* img, float* illum, hipStream_t s);
 

@@ -155,39 +155,43 @@ __device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest e
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float(((unsigned)b) << 16); }
 
-// Strip element index -> grid cell.  Slot layout: lf[mnz][nfd], rt[mnz][nfd],
-// up[nfd][mnx], dw[nfd][mnx] (Dlf/Drt/Dup/Ddw of kernel.cu:199-205).
-__device__ __forceinline__ void strip_cell(const Grid& g, long long e, int* z, int* x) {
-  const long long a = (long long)g.mnz * g.nfd, b = (long long)g.nfd * g.mnx;
+// Strip element index -> grid cell of a shot of width nx (interior mnx).
+// Slot layout: lf[mnz][nfd], rt[mnz][nfd], up[nfd][mnx], dw[nfd][mnx]
+// (Dlf/Drt/Dup/Ddw of kernel.cu:199-205); a slot holds ss_of(mnx) elements.
+__device__ __forceinline__ long long strip_size(const Grid& g, int mnx) {
+  return 2LL * g.nfd * (g.mnz + mnx);
+}
+__device__ __forceinline__ void strip_cell(const Grid& g, int nx, int mnx, long long e, int* z, int* x) {
+  const long long a = (long long)g.mnz * g.nfd, b = (long long)g.nfd * mnx;
   if (e < a) {
     int r = (int)(e / g.nfd), d = (int)(e % g.nfd);
     *z = g.n2 + r; *x = g.n2 - 1 - d;
   } else if (e < 2 * a) {
     e -= a;
     int r = (int)(e / g.nfd), d = (int)(e % g.nfd);
-    *z = g.n2 + r; *x = g.nx - g.n2 + d;
+    *z = g.n2 + r; *x = nx - g.n2 + d;
   } else if (e < 2 * a + b) {
     e -= 2 * a;
-    int d = (int)(e / g.mnx), c = (int)(e % g.mnx);
+    int d = (int)(e / mnx), c = (int)(e % mnx);
     *z = g.n2 - 1 - d; *x = g.n2 + c;
   } else {
     e -= 2 * a + b;
-    int d = (int)(e / g.mnx), c = (int)(e % g.mnx);
+    int d = (int)(e / mnx), c = (int)(e % mnx);
     *z = g.nz - g.n2 + d; *x = g.n2 + c;
   }
 }
 // Grid cell -> strip element index, or -1.
-__device__ __forceinline__ long long strip_index(const Grid& g, int z, int x) {
-  const long long a = (long long)g.mnz * g.nfd, b = (long long)g.nfd * g.mnx;
+__device__ __forceinline__ long long strip_index(const Grid& g, int nx, int mnx, int z, int x) {
+  const long long a = (long long)g.mnz * g.nfd, b = (long long)g.nfd * mnx;
   const int n2 = g.n2, nfd = g.nfd;
   if (z >= n2 && z < g.nz - n2) {
     if (x >= n2 - nfd && x < n2) return (long long)(z - n2) * nfd + (n2 - 1 - x);
-    if (x >= g.nx - n2 && x < g.nx - n2 + nfd) return a + (long long)(z - n2) * nfd + (x - (g.nx - n2));
+    if (x >= nx - n2 && x < nx - n2 + nfd) return a + (long long)(z - n2) * nfd + (x - (nx - n2));
     return -1;
   }
-  if (x >= n2 && x < g.nx - n2) {
-    if (z >= n2 - nfd && z < n2) return 2 * a + (long long)(n2 - 1 - z) * g.mnx + (x - n2);
-    if (z >= g.nz - n2 && z < g.nz - n2 + nfd) return 2 * a + b + (long long)(z - (g.nz - n2)) * g.mnx + (x - n2);
+  if (x >= n2 && x < nx - n2) {
+    if (z >= n2 - nfd && z < n2) return 2 * a + (long long)(n2 - 1 - z) * mnx + (x - n2);
+    if (z >= g.nz - n2 && z < g.nz - n2 + nfd) return 2 * a + b + (long long)(z - (g.nz - n2)) * mnx + (x - n2);
   }
   return -1;
 }
@@ -262,14 +266,15 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   const int shot = (int)(blockIdx.x % (unsigned)a.nshot);
   const int tile = (int)(blockIdx.x / (unsigned)a.nshot);
   const int xt = tile % a.nxt, zg = tile / a.nxt;
+  const ShotGeo sg = a.geo[shot];
   const int zhi = g.nz - g.n2;
   const int zs = g.n2 + (zg * 4 + wv) * a.zs;  // a.zs is a multiple of QL
-  if (zs >= zhi) return;
-  const int ze = zs + a.zs;
-  const int cend = g.nx - g.n2;                 // first column past the interior
+  const int cend = sg.nx - g.n2;                // first column past the shot's interior
   const int cb = g.n2 + xt * 256;
+  if (zs >= zhi || cb >= cend) return;          // tiles past a narrower shot's width
+  const int ze = zs + a.zs;
   const int c0 = cb + lane * 4;
-  const int clast = g.nx - ((g.nx - g.n2) & 3); // largest c == n2 (mod 4) with c <= nx
+  const int clast = sg.nx - ((sg.nx - g.n2) & 3);  // largest c == n2 (mod 4) with c <= nx
   const int cl = min(c0, clast);
   const unsigned vcol = (unsigned)(g.x0 + cl) * 4u;
   const unsigned vst = c0 < cend ? (unsigned)(g.x0 + c0) * 4u : kDrop;
@@ -288,12 +293,16 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   const long long so = (long long)shot * g.fstride;
   const unsigned fb = (unsigned)g.fstride * 4u;
   const Rsrc R1 = rsrc(a.u1 + so, fb), R0 = rsrc(a.u0 + so, fb), R2 = rsrc(a.u2 + so, fb);
-  const Rsrc RP = rsrc(m.pm, fb), RK = rsrc(m.ko, fb);
+  const long long mo = (long long)shot * m.mstride;
+  const Rsrc RP = rsrc(m.pm + mo, fb), RK = rsrc(m.ko + mo, fb);
   const Rsrc RC = rsrc(m.ctab, (unsigned)(m.nrow * m.cs) * 4u);
   const unsigned P4 = (unsigned)g.pitch * 4u;
   auto rowb = [&](int z) -> unsigned { return (unsigned)min(z, g.nz - 1) * P4; };
 
-  const int2 sp = a.src ? a.src[shot] : make_int2(-1, -1);
+  const int2 sp = make_int2(sg.r_u, sg.r_x);
+  Rcv rr = a.r;  // this shot's receivers; a.r.n stays the stride of the data layout
+  rr.s_l = sg.s_l;
+  rr.s_r = sg.s_r;
   bool srcc[4];
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) srcc[jj] = c0 + jj == sp.y;
@@ -304,7 +313,7 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
       int s;
-      if (colok[jj] && is_receiver(c0 + jj, a.r, &s))
+      if (colok[jj] && is_receiver(c0 + jj, rr, &s))
         dval[jj] = a.data[((long long)shot * a.nt + (a.k + 1)) * a.r.n + s];
     }
   }
@@ -442,9 +451,12 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   __shared__ float sT[TMAX];
   const Grid& g = a.g;
   const Model& m = a.m;
-  // shot-fastest block order, as in k_interior
-  const int shot = (int)(blockIdx.x % (unsigned)a.nshot);
-  const int4 td = a.tiles[blockIdx.x / (unsigned)a.nshot];
+  // One block per (tile, shot) record; the host lists them tile-major with the
+  // shot fastest, the XCD-friendly order of k_interior.
+  const int4 td = a.tiles[blockIdx.x];
+  const int shot = td.w;
+  const ShotGeo sg = a.geo[shot];
+  const int NX = sg.nx;  // this shot's grid width
   const int band = td.z;
   const int TZ = band < 2 ? 32 : 64, TX = band < 2 ? 64 : 32;
   const int RZ = TZ + 2 + 2 * NFD, RX = TX + 2 + 2 * NFD;
@@ -454,15 +466,21 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   const float* __restrict__ U1 = a.u1 + so;
   const float* __restrict__ U0 = a.u0 + so;
   float* __restrict__ U2 = a.u2 + so;
-  const float* __restrict__ V = m.v + g.x0;
-  const float* __restrict__ R1 = m.r1 + g.x0;
+  const long long mo = (long long)shot * m.mstride + g.x0;
+  const float* __restrict__ V = m.v + mo;
+  const float* __restrict__ R1 = m.r1 + mo;
+  const float* __restrict__ PM = m.pm + mo;
+  const unsigned* __restrict__ KO = m.ko + mo;
   const int P = g.pitch;
-  const int2 sp = a.src ? a.src[shot] : make_int2(-1, -1);
+  const int2 sp = make_int2(sg.r_u, sg.r_x);
+  Rcv rr = a.r;
+  rr.s_l = sg.s_l;
+  rr.s_r = sg.s_r;
 
   for (TileWalk t(RX); t.i < RZ; t.next()) {
     const int i = t.i, jx = t.j, idx = i * RX + jx;
     int gz = mirror_idx(z0 - 1 - NFD + i, g.nz);
-    int gx = mirror_idx(x0 - 1 - NFD + jx, g.nx);
+    int gx = mirror_idx(x0 - 1 - NFD + jx, NX);
     su1[idx] = U1[(long)gz * P + gx];
   }
   __syncthreads();
@@ -471,11 +489,11 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
     const int i = t.i, jx = t.j, idx = i * TRX + jx;
     int gz = z0 - 1 + i, gx = x0 - 1 + jx;
     float T = 0.f;
-    if (gz >= 0 && gz < g.nz && gx >= 0 && gx < g.nx) {
+    if (gz >= 0 && gz < g.nz && gx >= 0 && gx < NX) {
       const int ci = (i + NFD) * RX + (jx + NFD);
       const long long pc = (long)gz * P + gx;
       CoefRow<NFD> cr;
-      cr.load(m.ctab, m.ko[g.x0 + pc]);
+      cr.load(m.ctab, KO[pc]);
       const float u = su1[ci];
       float w1 = (float)(cr.c0d() * (double)u);
 #pragma unroll
@@ -487,16 +505,16 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
       }
       const float u0 = U0[pc];
       if (MODE == FR_FWD) {
-        T = combine_p(u, u0, m.pm[g.x0 + pc], w1, a.f64comb != 0);
+        T = combine_p(u, u0, PM[pc], w1, a.f64comb != 0);
         if (gz == sp.x && gx == sp.y) T = T + a.wav;
       } else {
         int s;
         bool inj = false;
-        if (gz == a.r.s_z && is_receiver(gx, a.r, &s)) {
+        if (gz == a.r.s_z && is_receiver(gx, rr, &s)) {
           float d = a.data[((long long)shot * a.nt + (a.k + 1)) * a.r.n + s];
           if (d != 0.0f) { T = d; inj = true; }
         }
-        if (!inj) T = combine_p(u, u0, m.pm[g.x0 + pc], w1, false);
+        if (!inj) T = combine_p(u, u0, PM[pc], w1, false);
       }
     }
     sT[idx] = T;
@@ -507,13 +525,13 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
     const int i = t.i, jx = t.j;
     const int z = z0 + i, x = x0 + jx;
     bool in;
-    if (band == 0) in = z < g.n2 && x < g.nx;
-    else if (band == 1) in = z >= g.nz - g.n2 && z < g.nz && x < g.nx;
+    if (band == 0) in = z < g.n2 && x < NX;
+    else if (band == 1) in = z >= g.nz - g.n2 && z < g.nz && x < NX;
     else if (band == 2) in = z >= g.n2 && z < g.nz - g.n2 && x < g.n2;
-    else in = z >= g.n2 && z < g.nz - g.n2 && x >= g.nx - g.n2 && x < g.nx;
+    else in = z >= g.n2 && z < g.nz - g.n2 && x >= NX - g.n2 && x < NX;
     if (!in) continue;
-    const int ar = min(min(z, x), min(g.nz - 1 - z, g.nx - 1 - x));  // ring
-    const int br = g.nz - 1 - ar, crr = g.nx - 1 - ar;
+    const int ar = min(min(z, x), min(g.nz - 1 - z, NX - 1 - x));  // ring
+    const int br = g.nz - 1 - ar, crr = NX - 1 - ar;
     const long long pg = (long)z * P + x;
     auto Tat = [&](int dz, int dx) { return sT[(i + 1 + dz) * TRX + (jx + 1 + dx)]; };
     auto U1at = [&](int dz, int dx) { return su1[(i + 1 + NFD + dz) * RX + (jx + 1 + NFD + dx)]; };
@@ -537,7 +555,7 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
         dxi = (x == ar) ? 1 : -1;
         // Dv[(N2-y)*NX+x] as a linear index: row ar + z/nx, column z%nx
         int zq = ar, xq = z;
-        if (xq >= g.nx) { zq += xq / g.nx; xq -= (xq / g.nx) * g.nx; }
+        if (xq >= NX) { zq += xq / NX; xq -= (xq / NX) * NX; }
         vq = V[(long)zq * P + xq];
       }
       const long long pi = pg + (long)dzi * P + dxi;
@@ -559,7 +577,7 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
     U2[pg] = val;
     if (MODE == FR_FWD) {
       if (a.save_strips) {
-        long long e = strip_index(g, z, x);
+        long long e = strip_index(g, NX, sg.mnx, z, x);
         if (e >= 0) strip_put(a.st, shot, a.k, e, val);
       }
     }
@@ -575,13 +593,13 @@ __global__ void k_fill(float* p, long long n, float val) {
 }
 
 // u(0) = 0, u(1) = f(0)/2 at the source (kernel.cu:798-806).
-__global__ void k_fwd_begin(Grid g, float* f0, float* f1, const int2* src, float spike) {
+__global__ void k_fwd_begin(Grid g, float* f0, float* f1, const ShotGeo* geo, float spike) {
   const int shot = blockIdx.y;
   const long long n = (long long)g.nz * g.pitch;
   float* a = f0 + (long long)shot * g.fstride;
   float* b = f1 + (long long)shot * g.fstride;
-  const int2 sp = src[shot];
-  const long long ps = (long long)sp.x * g.pitch + g.x0 + sp.y;
+  const ShotGeo sg = geo[shot];
+  const long long ps = (long long)sg.r_u * g.pitch + g.x0 + sg.r_x;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     a[i] = 0.0f;
@@ -589,69 +607,78 @@ __global__ void k_fwd_begin(Grid g, float* f0, float* f1, const int2* src, float
   }
 }
 
-__global__ void k_strip_save(Grid g, const float* f, Strips st, int k) {
+__global__ void k_strip_save(Grid g, const float* f, Strips st, const ShotGeo* geo, int k) {
   const int shot = blockIdx.y;
+  const ShotGeo sg = geo[shot];
+  const long long ss = strip_size(g, sg.mnx);
   const float* F = f + (long long)shot * g.fstride + g.x0;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < st.ss;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ss;
        e += (long long)gridDim.x * blockDim.x) {
     int z, x;
-    strip_cell(g, e, &z, &x);
+    strip_cell(g, sg.nx, sg.mnx, e, &z, &x);
     strip_put(st, shot, k, e, F[(long)z * g.pitch + x]);
   }
 }
 
 // Quantise the staged fp32 strips of slot k with the slot's power-of-two scale.
-__global__ void k_strip_finish(Grid g, Strips st, int k) {
+__global__ void k_strip_finish(Grid g, Strips st, const ShotGeo* geo, int k) {
   const int shot = blockIdx.y;
+  const long long ss = strip_size(g, geo[shot].mnx);
   const long long slot = (long long)shot * st.nslot + (k % st.nslot);
   const float sc = pow2_scale(__uint_as_float(st.amax[slot]));
   if (blockIdx.x == 0 && threadIdx.x == 0) st.scale[slot] = sc;
   const float inv = 1.0f / sc;     // exact: sc is a power of two
   const float* s = st.stage + ((long long)shot * 2 + (k & 1)) * st.ss;
   unsigned short* d = (unsigned short*)st.base + slot * st.ss;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < st.ss;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ss;
        e += (long long)gridDim.x * blockDim.x) {
     float x = s[e] * inv;
     d[e] = (st.fmt == 1) ? __half_as_ushort(__float2half_rn(x)) : f2bf(x);
   }
 }
 
-__global__ void k_strip_restore(Grid g, float* f, Strips st, int k) {
+__global__ void k_strip_restore(Grid g, float* f, Strips st, const ShotGeo* geo, int k) {
   const int shot = blockIdx.y;
+  const ShotGeo sg = geo[shot];
+  const long long ss = strip_size(g, sg.mnx);
   float* F = f + (long long)shot * g.fstride + g.x0;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < st.ss;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ss;
        e += (long long)gridDim.x * blockDim.x) {
     int z, x;
-    strip_cell(g, e, &z, &x);
+    strip_cell(g, sg.nx, sg.mnx, e, &z, &x);
     F[(long)z * g.pitch + x] = strip_get(st, shot, k, e);
   }
 }
 
 // ---- checkpoint snapshots -------------------------------------------------
+// A snapshot slot holds nz*nx (widest shot) elements; shot s uses the first
+// nz*nx_s of them, row-major over its own width.
 
 __global__ void k_snap_clear(Snaps sn, int slot, int nshot) {  // a kernel, not a memset: capturable
   const int sh = blockIdx.x * blockDim.x + threadIdx.x;
   if (sh < nshot) sn.amax[(long long)sh * sn.nsnap + slot] = 0u;
 }
 
-__global__ void k_snap_amax(Grid g, const float* f, Snaps sn, int slot) {
+__global__ void k_snap_amax(Grid g, const float* f, Snaps sn, const ShotGeo* geo, int slot) {
   const int shot = blockIdx.y;
+  const int nx = geo[shot].nx;
   const float* F = f + (long long)shot * g.fstride + g.x0;
-  const long long n = (long long)g.nz * g.nx;
+  const long long n = (long long)g.nz * nx;
   float mx = 0.0f;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x) {
-    const int z = (int)(e / g.nx), x = (int)(e % g.nx);
+    const int z = (int)(e / nx), x = (int)(e % nx);
     mx = fmaxf(mx, fabsf(F[(long)z * g.pitch + x]));
   }
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
   if ((threadIdx.x & 63) == 0) atomicMax(&sn.amax[(long long)shot * sn.nsnap + slot], __float_as_uint(mx));
 }
 
-__global__ void k_snap_save(Grid g, const float* f, Snaps sn, int slot) {
+__global__ void k_snap_save(Grid g, const float* f, Snaps sn, const ShotGeo* geo, int slot) {
   const int shot = blockIdx.y;
+  const int nx = geo[shot].nx;
   const float* F = f + (long long)shot * g.fstride + g.x0;
-  const long long n = (long long)g.nz * g.nx;
+  const long long n = (long long)g.nz * nx, stride = (long long)g.nz * g.nx;
   const long long so = (long long)shot * sn.nsnap + slot;
   float inv = 1.0f;
   if (sn.fmt != 0) {
@@ -661,54 +688,58 @@ __global__ void k_snap_save(Grid g, const float* f, Snaps sn, int slot) {
   }
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x) {
-    const int z = (int)(e / g.nx), x = (int)(e % g.nx);
+    const int z = (int)(e / nx), x = (int)(e % nx);
     const float v = F[(long)z * g.pitch + x];
     if (sn.fmt == 0) {
-      ((float*)sn.base)[so * n + e] = v;
+      ((float*)sn.base)[so * stride + e] = v;
     } else {
       const float y = v * inv;
-      ((unsigned short*)sn.base)[so * n + e] =
+      ((unsigned short*)sn.base)[so * stride + e] =
           (sn.fmt == 1) ? __half_as_ushort(__float2half_rn(y)) : f2bf(y);
     }
   }
 }
 
-__global__ void k_snap_restore(Grid g, float* f, Snaps sn, int slot) {
+__global__ void k_snap_restore(Grid g, float* f, Snaps sn, const ShotGeo* geo, int slot) {
   const int shot = blockIdx.y;
+  const int nx = geo[shot].nx;
   float* F = f + (long long)shot * g.fstride + g.x0;
-  const long long n = (long long)g.nz * g.nx;
+  const long long n = (long long)g.nz * nx, stride = (long long)g.nz * g.nx;
   const long long so = (long long)shot * sn.nsnap + slot;
   const float sc = sn.fmt != 0 ? sn.scale[so] : 1.0f;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x) {
-    const int z = (int)(e / g.nx), x = (int)(e % g.nx);
+    const int z = (int)(e / nx), x = (int)(e % nx);
     float v;
     if (sn.fmt == 0) {
-      v = ((const float*)sn.base)[so * n + e];
+      v = ((const float*)sn.base)[so * stride + e];
     } else {
-      const unsigned short h = ((const unsigned short*)sn.base)[so * n + e];
+      const unsigned short h = ((const unsigned short*)sn.base)[so * stride + e];
       v = (sn.fmt == 1 ? __half2float(__ushort_as_half(h)) : bf2f(h)) * sc;
     }
     F[(long)z * g.pitch + x] = v;
   }
 }
 
-__global__ void k_record(Grid g, Rcv r, const float* f, float* rec, int k, int nt) {
+// Receiver recording of u(k): rec[shot][k][s], s < n of the shot (stride r.n).
+__global__ void k_record(Grid g, Rcv r, const ShotGeo* geo, const float* f, float* rec, int k,
+                         int nt) {
   const int shot = blockIdx.y;
+  const ShotGeo sg = geo[shot];
   const float* F = f + (long long)shot * g.fstride + g.x0;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < r.n; s += gridDim.x * blockDim.x)
-    rec[((long long)shot * nt + k) * r.n + s] = F[(long)r.s_z * g.pitch + r.s_l + s * r.ds];
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < sg.n; s += gridDim.x * blockDim.x)
+    rec[((long long)shot * nt + k) * r.n + s] = F[(long)r.s_z * g.pitch + sg.s_l + s * r.ds];
 }
 
 // Backward initial state (kernel.cu:859-885).  s0 = u(nt-1) (DBW0),
 // s1 = u(nt-2) (DBW1); FW0 = 0 and FW1 = spike field are host arrays there.
 __global__ void k_bwd_begin(Grid g, const float* s0, const float* s1, float* q0, float* q1,
-                            float* su, float* sq, float* i1, float* i2, const int2* src,
+                            float* su, float* sq, float* i1, float* i2, const ShotGeo* geo,
                             float spike, int icompen) {
   const int shot = blockIdx.y;
   const long long base = (long long)shot * g.fstride;
-  const int2 sp = src[shot];
-  const long long ps = (long long)sp.x * g.pitch + g.x0 + sp.y;
+  const ShotGeo sg = geo[shot];
+  const long long ps = (long long)sg.r_u * g.pitch + g.x0 + sg.r_x;
   const long long n = (long long)g.nz * g.pitch;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
@@ -733,56 +764,68 @@ __global__ void k_bwd_begin(Grid g, const float* s0, const float* s1, float* q0,
 }
 
 // max |I2| over the interior, per shot (kernel.cu:963-970).
-__global__ void k_maxabs(Grid g, const float* i2, unsigned* out) {
+__global__ void k_maxabs(Grid g, const ShotGeo* geo, const float* i2, unsigned* out) {
   const int shot = blockIdx.y;
+  const int mnx = geo[shot].mnx;
   const float* I = i2 + (long long)shot * g.fstride + g.x0;
-  const long long n = (long long)g.mnz * g.mnx;
+  const long long n = (long long)g.mnz * mnx;
   float mx = 0.0f;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x) {
-    int z = g.n2 + (int)(e / g.mnx), x = g.n2 + (int)(e % g.mnx);
+    int z = g.n2 + (int)(e / mnx), x = g.n2 + (int)(e % mnx);
     mx = fmaxf(mx, fabsf(I[(long)z * g.pitch + x]));
   }
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
   if ((threadIdx.x & 63) == 0) atomicMax(&out[shot], __float_as_uint(mx));
 }
 
-// Per-shot image post (kernel.cu:935-979).
-__global__ void k_post(Grid g, const float* v, const float* i1, const float* i2,
-                       const unsigned* amax, float vmax, float whitecoe, float* up, float* down) {
+// Per-shot image post (kernel.cu:935-979).  up/down: [shot][mnz][g.mnx].
+__global__ void k_post(Grid g, const ShotGeo* geo, const float* v, long long mstride,
+                       const float* i1, const float* i2, const unsigned* amax, float vmax,
+                       float whitecoe, float* up, float* down) {
   const int shot = blockIdx.y;
+  const ShotGeo sg = geo[shot];
+  const int mnx = sg.mnx, nx = sg.nx;
   const float* M = i1 + (long long)shot * g.fstride + g.x0;
   const float* I = i2 + (long long)shot * g.fstride + g.x0;
-  const float* Vv = v + g.x0;
-  const long long n = (long long)g.mnz * g.mnx;
+  const float* Vv = v + (long long)shot * mstride + g.x0;
+  const long long n = (long long)g.mnz * mnx;
   const float stable = __uint_as_float(amax[shot]) * whitecoe;
   const int P = g.pitch, n2 = g.n2;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x) {
-    const int i = n2 + (int)(e / g.mnx), j = n2 + (int)(e % g.mnx);
+    const int zi = (int)(e / mnx), xi = (int)(e % mnx);
+    const int i = n2 + zi, j = n2 + xi;
     int i1_ = i - 1, i2_ = i + 1, j1 = j - 1, j2 = j + 1;
     if (i1_ < n2) i1_ = 2 * n2 - i1_;
     if (j1 < n2) j1 = 2 * n2 - j1;
     if (i2_ >= g.nz - n2) i2_ = 2 * (g.nz - n2 - 1) - i2_;
-    if (j2 >= g.nx - n2) j2 = 2 * (g.nx - n2 - 1) - j2;
+    if (j2 >= nx - n2) j2 = 2 * (nx - n2 - 1) - j2;
     const float sum = M[(long)i * P + j2] + M[(long)i * P + j1] + M[(long)i2_ * P + j] +
                       M[(long)i1_ * P + j] - 4.0f * M[(long)i * P + j];
     const float vv = Vv[(long)i * P + j];
     const double t = -1.0 * (double)sum * (double)vv * (double)vv / (double)(vmax * vmax);
-    const long long o = (long long)shot * n + e;
+    const long long o = ((long long)shot * g.mnz + zi) * g.mnx + xi;
     if (up) up[o] = (float)t;
     if (down) down[o] = I[(long)i * P + j] + stable;
   }
 }
 
-// Running stack in shot order (kernel.cu:1026-1039).
-__global__ void k_stack_add(int nI, const float* up, const float* down, float* s1, float* s2,
-                            int nshot) {
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nI; e += gridDim.x * blockDim.x) {
+// Running stack in shot order (kernel.cu:1026-1039) onto an image of gw
+// columns: shot s covers columns x_img .. x_img+mnx-1.
+__global__ void k_stack_add(Grid g, const ShotGeo* geo, int gw, const float* up, const float* down,
+                            float* s1, float* s2, int nshot) {
+  const long long nI = (long long)g.mnz * gw;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nI;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int z = (int)(e / gw), X = (int)(e % gw);
     float a = s1[e], b = s2[e];
     for (int s = 0; s < nshot; ++s) {
-      a += up[(long long)s * nI + e];
-      b += down[(long long)s * nI + e];
+      const int x = X - geo[s].x_img;
+      if (x < 0 || x >= geo[s].mnx) continue;
+      const long long o = ((long long)s * g.mnz + z) * g.mnx + x;
+      a += up[o];
+      b += down[o];
     }
     s1[e] = a;
     s2[e] = b;
@@ -794,7 +837,9 @@ __global__ void k_stack_finish(int nI, const float* s1, const float* s2, int nre
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nI; e += gridDim.x * blockDim.x) {
     float a = s1[e] / nrec;
     float b = s2[e] / nrec;
-    if (inorm == 1) a = a / b;
+    // a cell that no shot of a ragged batch covers (both sums 0) stays 0
+    // instead of 0/0; every other cell is divided as in the reference
+    if (inorm == 1 && !(s1[e] == 0.0f && s2[e] == 0.0f)) a = a / b;
     img[e] = a;
     ill[e] = b;
   }
@@ -829,9 +874,9 @@ hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_frame(int mode, int nfd, FrameArgs a, int nshot, int ntiles, hipStream_t s) {
+hipError_t launch_frame(int mode, int nfd, FrameArgs a, int nshot, int nrec, hipStream_t s) {
   a.nshot = nshot;
-  dim3 grid(ntiles * nshot), block(256);
+  dim3 grid(nrec), block(256);  // one block per (tile, shot) record
 #define CASE(N)                                                                          \
   case N:                                                                                \
     if (mode == FR_FWD) hipLaunchKernelGGL((k_frame<N, FR_FWD>), grid, block, 0, s, a);  \
@@ -855,69 +900,80 @@ hipError_t launch_fill(float* p, long long n, float val, hipStream_t s) {
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, s, p, n, val);
   return hipGetLastError();
 }
-hipError_t launch_fwd_begin(const Grid& g, float* f0, float* f1, const int2* src, float spike,
+hipError_t launch_fwd_begin(const Grid& g, float* f0, float* f1, const ShotGeo* geo, float spike,
                             int nshot, hipStream_t s) {
   hipLaunchKernelGGL(k_fwd_begin, dim3(blocks_for((long long)g.nz * g.pitch, 256, 1024), nshot),
-                     dim3(256), 0, s, g, f0, f1, src, spike);
+                     dim3(256), 0, s, g, f0, f1, geo, spike);
   return hipGetLastError();
 }
-hipError_t launch_strip_save(const Grid& g, const float* f, Strips st, int k, int nshot,
-                             hipStream_t s) {
-  hipLaunchKernelGGL(k_strip_save, dim3(blocks_for(st.ss), nshot), dim3(256), 0, s, g, f, st, k);
+static inline long long strip_size_host(const Grid& g) { return 2LL * g.nfd * (g.mnz + g.mnx); }
+hipError_t launch_strip_save(const Grid& g, const float* f, Strips st, const ShotGeo* geo, int k,
+                             int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_strip_save, dim3(blocks_for(strip_size_host(g)), nshot), dim3(256), 0, s, g,
+                     f, st, geo, k);
   return hipGetLastError();
 }
-hipError_t launch_strip_finish(const Grid& g, Strips st, int k, int nshot, hipStream_t s) {
-  hipLaunchKernelGGL(k_strip_finish, dim3(blocks_for(st.ss), nshot), dim3(256), 0, s, g, st, k);
+hipError_t launch_strip_finish(const Grid& g, Strips st, const ShotGeo* geo, int k, int nshot,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_strip_finish, dim3(blocks_for(strip_size_host(g)), nshot), dim3(256), 0, s,
+                     g, st, geo, k);
   return hipGetLastError();
 }
-hipError_t launch_strip_restore(const Grid& g, float* f, Strips st, int k, int nshot,
-                                hipStream_t s) {
-  hipLaunchKernelGGL(k_strip_restore, dim3(blocks_for(st.ss), nshot), dim3(256), 0, s, g, f, st, k);
+hipError_t launch_strip_restore(const Grid& g, float* f, Strips st, const ShotGeo* geo, int k,
+                                int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_strip_restore, dim3(blocks_for(strip_size_host(g)), nshot), dim3(256), 0, s,
+                     g, f, st, geo, k);
   return hipGetLastError();
 }
-hipError_t launch_snap_save(const Grid& g, const float* f, Snaps sn, int slot, int nshot,
-                            hipStream_t s) {
+hipError_t launch_snap_save(const Grid& g, const float* f, Snaps sn, const ShotGeo* geo, int slot,
+                            int nshot, hipStream_t s) {
   const long long n = (long long)g.nz * g.nx;
   if (sn.fmt != 0) {
     hipLaunchKernelGGL(k_snap_clear, dim3(blocks_for(nshot)), dim3(256), 0, s, sn, slot, nshot);
-    hipLaunchKernelGGL(k_snap_amax, dim3(blocks_for(n, 256, 512), nshot), dim3(256), 0, s, g, f, sn, slot);
+    hipLaunchKernelGGL(k_snap_amax, dim3(blocks_for(n, 256, 512), nshot), dim3(256), 0, s, g, f, sn,
+                       geo, slot);
   }
-  hipLaunchKernelGGL(k_snap_save, dim3(blocks_for(n, 256, 1024), nshot), dim3(256), 0, s, g, f, sn, slot);
+  hipLaunchKernelGGL(k_snap_save, dim3(blocks_for(n, 256, 1024), nshot), dim3(256), 0, s, g, f, sn,
+                     geo, slot);
   return hipGetLastError();
 }
-hipError_t launch_snap_restore(const Grid& g, float* f, Snaps sn, int slot, int nshot,
-                               hipStream_t s) {
+hipError_t launch_snap_restore(const Grid& g, float* f, Snaps sn, const ShotGeo* geo, int slot,
+                               int nshot, hipStream_t s) {
   const long long n = (long long)g.nz * g.nx;
-  hipLaunchKernelGGL(k_snap_restore, dim3(blocks_for(n, 256, 1024), nshot), dim3(256), 0, s, g, f, sn, slot);
+  hipLaunchKernelGGL(k_snap_restore, dim3(blocks_for(n, 256, 1024), nshot), dim3(256), 0, s, g, f,
+                     sn, geo, slot);
   return hipGetLastError();
 }
-hipError_t launch_record(const Grid& g, const Rcv& r, const float* f, float* rec, int k, int nt,
-                         int nshot, hipStream_t s) {
-  hipLaunchKernelGGL(k_record, dim3(blocks_for(r.n), nshot), dim3(256), 0, s, g, r, f, rec, k, nt);
+hipError_t launch_record(const Grid& g, const Rcv& r, const ShotGeo* geo, const float* f,
+                         float* rec, int k, int nt, int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_record, dim3(blocks_for(r.n), nshot), dim3(256), 0, s, g, r, geo, f, rec, k,
+                     nt);
   return hipGetLastError();
 }
 hipError_t launch_bwd_begin(const Grid& g, const float* s0, const float* s1, float* q0, float* q1,
-                            float* su, float* sq, float* i1, float* i2, const int2* src,
+                            float* su, float* sq, float* i1, float* i2, const ShotGeo* geo,
                             float spike, int icompen, int nshot, hipStream_t s) {
   hipLaunchKernelGGL(k_bwd_begin, dim3(blocks_for((long long)g.nz * g.pitch, 256, 1024), nshot),
-                     dim3(256), 0, s, g, s0, s1, q0, q1, su, sq, i1, i2, src, spike, icompen);
+                     dim3(256), 0, s, g, s0, s1, q0, q1, su, sq, i1, i2, geo, spike, icompen);
   return hipGetLastError();
 }
-hipError_t launch_maxabs(const Grid& g, const float* i2, unsigned* out, int nshot, hipStream_t s) {
+hipError_t launch_maxabs(const Grid& g, const ShotGeo* geo, const float* i2, unsigned* out,
+                         int nshot, hipStream_t s) {
   hipLaunchKernelGGL(k_maxabs, dim3(blocks_for((long long)g.mnz * g.mnx, 256, 256), nshot),
-                     dim3(256), 0, s, g, i2, out);
+                     dim3(256), 0, s, g, geo, i2, out);
   return hipGetLastError();
 }
-hipError_t launch_post(const Grid& g, const float* v, const float* i1, const float* i2,
-                       const unsigned* amax, float vmax, float whitecoe, float* up, float* down,
-                       int nshot, hipStream_t s) {
+hipError_t launch_post(const Grid& g, const ShotGeo* geo, const float* v, long long mstride,
+                       const float* i1, const float* i2, const unsigned* amax, float vmax,
+                       float whitecoe, float* up, float* down, int nshot, hipStream_t s) {
   hipLaunchKernelGGL(k_post, dim3(blocks_for((long long)g.mnz * g.mnx, 256, 1024), nshot),
-                     dim3(256), 0, s, g, v, i1, i2, amax, vmax, whitecoe, up, down);
+                     dim3(256), 0, s, g, geo, v, mstride, i1, i2, amax, vmax, whitecoe, up, down);
   return hipGetLastError();
 }
-hipError_t launch_stack_add(int nI, const float* up, const float* down, float* s1, float* s2,
-                            int nshot, hipStream_t s) {
-  hipLaunchKernelGGL(k_stack_add, dim3(blocks_for(nI)), dim3(256), 0, s, nI, up, down, s1, s2, nshot);
+hipError_t launch_stack_add(const Grid& g, const ShotGeo* geo, int gw, const float* up,
+                            const float* down, float* s1, float* s2, int nshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_stack_add, dim3(blocks_for((long long)g.mnz * gw)), dim3(256), 0, s, g, geo,
+                     gw, up, down, s1, s2, nshot);
   return hipGetLastError();
 }
 hipError_t launch_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,

@@ -26,7 +26,12 @@ class RtmParams(C.Structure):
                                         "icompen", "s_l", "s_r", "s_z", "ds", "n")] + \
                [(n, C.c_float) for n in ("tao2", "h2", "taoh", "taoh2", "hzx2_1")] + \
                [(n, C.c_int) for n in ("max_shots", "strip_format", "record", "zseg",
-                                       "snap_every")]
+                                       "snap_every", "stack_nx")]
+
+
+class RtmShotGeom(C.Structure):
+    """One shot of a ragged batch (include/rtm_hip.h: rtm_shot_geom)."""
+    _fields_ = [(n, C.c_int) for n in ("nx", "r_u", "r_x", "s_l", "s_r", "n", "x_img")]
 
 
 class RtmError(RuntimeError):
@@ -55,6 +60,8 @@ def lib():
                                     C.c_float]
         L.rtm_set_wavelet.argtypes = [vp, _fp]
         L.rtm_set_shots.argtypes = [vp, C.c_int, _ip, _ip]
+        L.rtm_set_shot_geometry.argtypes = [vp, C.c_int, C.POINTER(RtmShotGeom)]
+        L.rtm_set_shot_model.argtypes = [vp, C.c_int, _fp, _fp]
         L.rtm_set_gathers.argtypes = [vp, _fp]
         L.rtm_gathers_from_records.argtypes = [vp]
         L.rtm_forward_begin.argtypes = [vp, C.c_int]
@@ -72,6 +79,7 @@ def lib():
         L.rtm_set_timing.argtypes = [vp, C.c_int]
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
+                  "rtm_set_shot_geometry", "rtm_set_shot_model",
                   "rtm_set_gathers", "rtm_gathers_from_records", "rtm_forward_begin", "rtm_forward_advance",
                   "rtm_backward_begin", "rtm_backward_advance", "rtm_migrate", "rtm_set_graph", "rtm_sync",
                   "rtm_read_field", "rtm_read_records", "rtm_read_strip", "rtm_image_post",
@@ -95,6 +103,7 @@ class Context:
             raise RtmError(f"rtm_create failed ({rc})")
         self.h = h
         self.nz, self.nx, self.n2 = p.nz, p.nx, p.n2
+        self.gw = p.stack_nx if p.stack_nx > 0 else p.nx - 2 * p.n2
         self.nshot = 0
 
     def _chk(self, rc, what):
@@ -130,6 +139,18 @@ class Context:
         r_x = np.ascontiguousarray(r_x, np.int32)
         self._chk(lib().rtm_set_shots(self.h, len(r_u), r_u, r_x), "rtm_set_shots")
         self.nshot = len(r_u)
+
+    def set_shot_geometry(self, geoms):
+        """geoms: sequence of dicts with nx, r_u, r_x, s_l, s_r, n, x_img."""
+        arr = (RtmShotGeom * len(geoms))(*[RtmShotGeom(**g) for g in geoms])
+        self._chk(lib().rtm_set_shot_geometry(self.h, len(geoms), arr), "rtm_set_shot_geometry")
+        self.nshot = len(geoms)
+
+    def set_shot_model(self, shot, v, r1):
+        """Model window of one shot of a ragged batch: v, r1 [nz][nx of the shot]."""
+        self._chk(lib().rtm_set_shot_model(self.h, int(shot), np.ascontiguousarray(v, np.float32).ravel(),
+                                           np.ascontiguousarray(r1, np.float32).ravel()),
+                  "rtm_set_shot_model")
 
     def set_gathers(self, seis):
         self._chk(lib().rtm_set_gathers(self.h, np.ascontiguousarray(seis, np.float32).ravel()),
@@ -185,12 +206,12 @@ class Context:
         return up, down
 
     def stack_finish(self, nrec, inorm):
-        mnz, mnx = self.nz - 2 * self.n2, self.nx - 2 * self.n2
-        img = np.zeros(mnz * mnx, np.float32)
-        ill = np.zeros(mnz * mnx, np.float32)
+        mnz = self.nz - 2 * self.n2
+        img = np.zeros(mnz * self.gw, np.float32)
+        ill = np.zeros(mnz * self.gw, np.float32)
         self._chk(lib().rtm_stack_finish(self.h, int(nrec), int(inorm), img, ill.ctypes.data),
                   "rtm_stack_finish")
-        return img.reshape(mnz, mnx), ill.reshape(mnz, mnx)
+        return img.reshape(mnz, self.gw), ill.reshape(mnz, self.gw)
 
     def set_timing(self, mask):
         """mask: bit set of kernel kinds (1 << K_*), 0 off, -1 all."""
