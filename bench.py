@@ -65,6 +65,145 @@ def build_problem(shots, nfdmax=10, n2=30, nt=4000):
     return dict(cfg=cfg, d=d, models=models, r_u=r_u, r_x=r_x, wav=wav, w=w)
 
 
+def build_apertures(nshot, seed=3, nt=5000):
+    """Surface shots with uneven apertures on the seeded 4001 x 801 model
+    (dx = dz = 10 m, Gaussian-random-field velocity 1500-4500 m/s with a
+    500 m correlation length plus 6 sharp reflectors).  Shot s covers model
+    columns [start, start + width) with width uniform in 600..2400, its source
+    10 m deep at the window centre and a receiver at every surface point of
+    the window.  15 Hz Ricker, dt = 0.8 ms, N2 = 30, nfdmax = 4."""
+    from rtm_gpu_amd import host, synth
+    mod_nz, mod_nx, h, tao, f0, n2, nfdmax = 801, 4001, 10.0, 0.0008, 15.0, 30, 4
+    true_raw = synth.grf_4001(seed=seed, mod_nz=mod_nz, mod_nx=mod_nx, h=h)
+    mig_raw = synth.smooth(true_raw, 4.0)
+    rng = np.random.default_rng(seed)
+    widths = rng.integers(600, 2401, nshot)
+    starts = np.array([int(rng.integers(0, mod_nx - w + 1)) for w in widths])
+    wmax = int(widths.max())
+    cfg = host.make_config(nfdmax=nfdmax, nfdmin=2, n2=n2, f0=f0, fmax=2 * f0 + 1, df=1.0,
+                           nthita=1000, eps=1e-5, dv=1.0, ilste=0, ifv=0, whitecoe=1e-4, hz=h,
+                           tao=tao, inorm=1, icompen=1, h=h, tao1=tao, mod_nz=mod_nz,
+                           mod_nx=wmax, nt1=nt, s_l1=1, s_z1=1, n=wmax, ds=1, r_x1=1,
+                           nrec=nshot, dr=1)
+    d = host.derive(cfg)
+    # one coefficient table for both models and every window
+    vt, _ = host.pad_model(true_raw, mod_nz, mod_nx, n2, 0, tao, h)
+    vm, _ = host.pad_model(mig_raw, mod_nz, mod_nx, n2, 0, tao, h)
+    vmin, vmax, nvel = host.vrange(np.concatenate([vt.ravel(), vm.ravel()]), 1.0)
+    need = host.bins_needed(vt, vmin, 1.0, nvel) | host.bins_needed(vm, vmin, 1.0, nvel)
+    M, index, c = host.design_ls(1000, nfdmax, 2, nvel, tao, h, 1.0, 1e-5, cfg.fmax, vmin, 1.0,
+                                 need, d.hzx, nthreads=min(16, os.cpu_count() or 1))
+    r_u = host.source_row(10.0, h, n2)
+    geoms = [dict(nx=int(w) + 2 * n2, r_u=r_u, r_x=n2 + int(w) // 2, s_l=n2, s_r=n2 + int(w) - 1,
+                  n=int(w), x_img=int(xb)) for w, xb in zip(widths, starts)]
+
+    def window(raw, i):
+        xb, w = int(starts[i]), int(widths[i])
+        return host.pad_model(np.ascontiguousarray(raw[xb:xb + w]), mod_nz, w, n2, 0, tao, h)
+
+    return dict(cfg=cfg, d=d, geoms=geoms, widths=widths, starts=starts, true=true_raw,
+                mig=mig_raw, window=window, c=c, index=index, nvel=nvel, vmin=vmin, vmax=vmax,
+                wav=host.wavelet_table(d.nt, tao, f0), w=host.blend_weights(n2), n2=n2,
+                mod_nx=mod_nx, wmax=wmax)
+
+
+def run_apertures(a):
+    """Ragged batches of uneven-aperture shots against one shot per launch on a
+    sample of the same shots; one JSON line.  Both run eager: each batch is
+    migrated once, so a captured graph (rtm_set_graph, for repeated
+    migrations of one batch) would only add its capture time."""
+    from rtm_gpu_amd import hip
+    t_setup = time.time()
+    A = build_apertures(a.shots, nt=a.nt)
+    d, cfg, n2 = A["d"], A["cfg"], A["n2"]
+    log(f"[apertures] {a.shots} shots, widths {A['widths'].min()}..{A['widths'].max()}, "
+        f"{A['nvel']} velocity bins, setup {time.time() - t_setup:.1f}s")
+    common = dict(nz=d.nz, n2=n2, nfdmax=cfg.nfdmax, nt=d.nt, nt2=d.nt2, ilste=0, icompen=1,
+                  s_z=d.s_z, ds=1, tao2=d.tao2, h2=d.h2, taoh=d.taoh, taoh2=d.taoh2,
+                  hzx2_1=d.hzx2_1, strip_format={"f32": 0, "f16": 1, "bf16": 2}[a.strips])
+
+    def table(ctx, nx):
+        flat = np.full((d.nz, nx), A["vmin"], np.float32)
+        ctx.set_model(flat, flat * 0, A["w"], A["c"], A["index"], A["nvel"], A["vmin"], 1.0)
+
+    B = min(a.batch, a.shots)
+    nxw = A["wmax"] + 2 * n2
+    ctx = hip.Context(**common, nx=nxw, s_l=n2, s_r=n2 + A["wmax"] - 1, n=A["wmax"],
+                      max_shots=B, record=1, stack_nx=A["mod_nx"], zseg=a.zseg)
+    log(f"[apertures] batch context: {ctx.device_bytes() / 2**30:.1f} GiB")
+    ctx.set_wavelet(A["wav"])
+    timed, upd, zseg = 0.0, 0, a.zseg
+    for b0 in range(0, a.shots, B):
+        idx = list(range(b0, min(a.shots, b0 + B)))
+        table(ctx, nxw)
+        ctx.set_shot_geometry([A["geoms"][i] for i in idx])
+        for j, i in enumerate(idx):
+            v, r1 = A["window"](A["true"], i)
+            ctx.set_shot_model(j, v, r1)
+        ctx.forward_begin(save_strips=False)
+        ctx.forward_advance(d.nt)
+        ctx.gathers_from_records()
+        for j, i in enumerate(idx):
+            v, r1 = A["window"](A["mig"], i)
+            ctx.set_shot_model(j, v, r1)
+        if zseg <= 0:  # the tuner's forward steps leave the uploaded gathers alone
+            zseg = ctx.autotune(4)
+            log(f"[apertures] autotune: {zseg} rows per wave")
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.migrate()
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        timed += dt
+        for i in idx:
+            w = int(A["widths"][i])
+            nxs, nI = w + 2 * n2, (d.nz - 2 * n2) * w
+            upd += (d.nt - 2) * (2 * d.nz * nxs + nI)
+        ctx.image_post(A["vmax"], cfg.whitecoe, want=False)
+        log(f"[apertures] batch {b0 // B + 1}: {len(idx)} shots migrated in {dt:.2f}s")
+    img, _ = ctx.stack_finish(a.shots, 1)
+    assert np.isfinite(img).all(), "non-finite image"
+    ctx.close()
+    # one shot per launch: each sampled shot alone on a context of its own width
+    sample = sorted(set(np.linspace(0, a.shots - 1, a.sample).astype(int).tolist()))
+    single = 0.0
+    for i in sample:
+        gm = A["geoms"][i]
+        one = hip.Context(**common, nx=gm["nx"], s_l=gm["s_l"], s_r=gm["s_r"], n=gm["n"],
+                          max_shots=1, record=1, zseg=zseg)
+        v, r1 = A["window"](A["true"], i)
+        one.set_model(v, r1, A["w"], A["c"], A["index"], A["nvel"], A["vmin"], 1.0)
+        one.set_wavelet(A["wav"])
+        one.set_shots([gm["r_u"]], [gm["r_x"]])
+        one.forward_begin(save_strips=False)
+        one.forward_advance(d.nt)
+        one.gathers_from_records()
+        v, r1 = A["window"](A["mig"], i)
+        one.set_model(v, r1, A["w"], A["c"], A["index"], A["nvel"], A["vmin"], 1.0)
+        one.sync()
+        t0 = time.perf_counter()
+        one.migrate()
+        one.sync()
+        single += time.perf_counter() - t0
+        one.close()
+    single_sps = len(sample) / single
+    value = a.shots / timed
+    out = {"metric": "shots/s, ragged batch vs one shot per launch", "value": round(value, 3),
+           "unit": "shots/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": round(timed * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded GRF model with reflectors; gathers modelled by the same engine)",
+           "config": {"workload": f"grf_4001x801_apertures_{a.shots}shots", "nt": d.nt,
+                      "dt": 0.0008, "f0": 15.0, "n2": n2, "nfdmax": cfg.nfdmax,
+                      "aperture_columns": [int(A["widths"].min()), int(A["widths"].max())],
+                      "shots_per_launch": B, "rows_per_wave": zseg, "strips": a.strips},
+           "gpt_updates_per_s": round(upd / timed / 1e9, 3),
+           "one_shot_per_launch": {"shots_per_s": round(single_sps, 3),
+                                   "sample_shots": len(sample)},
+           "speedup_vs_one_shot_per_launch": round(value / single_sps, 3)}
+    print(json.dumps(out), flush=True)
+
+
 def algorithmic_bytes(kind, d, n2, nfd, nshot, n):
     """Minimum HBM bytes one launch of each kernel must move.  Per interior
     cell and shot: forward / source reconstruction read u1, u0 and write u2
@@ -161,6 +300,12 @@ def main():
     ap.add_argument("--strips", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--zseg", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="rvsp", choices=["rvsp", "apertures"],
+                    help="rvsp: the headline reverse-VSP run; apertures: uneven-aperture "
+                         "surface shots in ragged batches vs one shot per launch")
+    ap.add_argument("--batch", type=int, default=128, help="apertures: shots per launch")
+    ap.add_argument("--sample", type=int, default=8,
+                    help="apertures: shots timed one per launch")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe(a.copy_probe)
@@ -170,6 +315,13 @@ def main():
               "GPUs with one image all-reduce at the end, which this build does not run.",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "apertures":
+        if a.shots == 128 and "--shots" not in sys.argv:
+            a.shots = 256
+        if a.nt == 4000 and "--nt" not in sys.argv:
+            a.nt = 5000
+        run_apertures(a)
+        return
 
     from rtm_gpu_amd import hip
     t_setup = time.time()
@@ -191,6 +343,11 @@ def main():
     ctx.gathers_from_records()
     ctx.sync()
     ctx.set_model(mm["v"], mm["r1"], P["w"], mm["c"], mm["index"], mm["nvel"], mm["vmin"], 1.0)
+    zseg = a.zseg
+    if zseg <= 0:
+        t = time.time()
+        zseg = ctx.autotune(4)
+        log(f"[bench] autotune: {zseg} rows per wave ({time.time() - t:.1f}s)")
     log(f"[bench] setup + modelling {time.time() - t_setup:.1f}s")
 
     def step():
@@ -245,7 +402,7 @@ def main():
         "config": {"workload": workload, "grid": [d.nz, d.nx], "interior": [d.nz - 60, d.nx - 60],
                    "nt": d.nt, "shots": a.shots, "nfdmax": cfg.nfdmax, "n2": cfg.n2,
                    "strips": a.strips, "coefficients": "adaptive LS",
-                   "velocity_bins": mm["nbins"]},
+                   "velocity_bins": mm["nbins"], "rows_per_wave": zseg},
         "shots_per_s": round(a.shots * a.steps / el, 3),
         "hbm_frac_whole_job": round(total_bytes / el / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(kd["gbs"], 1),

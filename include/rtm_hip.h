@@ -155,13 +155,25 @@ int rtm_read_records(rtm_ctx* ctx, float* out);
 int rtm_read_strip(rtm_ctx* ctx, int shot, int k, float* out);
 
 /* Per-shot image post (kernel.cu:935-990): up = -lap(I1)*v^2/vmax^2,
- * down = I2 + whitecoe*max|I2|, both interior [mnz][mnx] row-major, for every
- * shot of the batch, written to up[shot][..], down[shot][..] when non-NULL,
- * and added in shot order to the context's stack (kernel.cu:1000-1040). */
+ * down = I2 + whitecoe*max|I2|, both interior [mnz][mnx] row-major (mnx of
+ * the context's grid; a narrower shot of a ragged batch fills its first
+ * columns), for every shot of the batch, written to up[shot][..],
+ * down[shot][..] when non-NULL, and added in shot order to the context's
+ * stack (kernel.cu:1000-1040) at the shot's image column. */
 int rtm_image_post(rtm_ctx* ctx, float vmax, float whitecoe, float* up, float* down);
 /* Stack result (kernel.cu:1042-1059): img = sum(up)/nrec, illum = sum(down)/nrec,
- * img /= illum when inorm = 1.  Interior [mnz][mnx].  Resets the stack. */
+ * img /= illum when inorm = 1.  [mnz][stack_nx] (stack_nx = mnx unless set).
+ * Resets the stack. */
 int rtm_stack_finish(rtm_ctx* ctx, int nrec, int inorm, float* img, float* illum);
+
+/* Autotuner (not in the reference): times nsteps forward steps of the
+ * current batch for each candidate height of the interior kernels' row
+ * segments (the kernels round it to a multiple of their register-queue
+ * length), keeps the fastest for this context and returns it in *zseg.
+ * Needs model, wavelet and shots; leaves the forward pass to be begun anew.
+ * Rows per wave trade the vertical halo re-read (2*NFD rows per segment)
+ * against the number of waves in flight. */
+int rtm_autotune(rtm_ctx* ctx, int nsteps, int* zseg);
 
 /* Kernel timing with HIP events on the context stream: mask is a bit set of
  * kernel kinds (1 << RTM_K_*), 0 = off, -1 = all.  Resets the counters. */

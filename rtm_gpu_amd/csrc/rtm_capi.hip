@@ -797,6 +797,48 @@ int rtm_set_graph(rtm_ctx* c, int on) {
   return RTM_OK;
 }
 
+int rtm_autotune(rtm_ctx* c, int nsteps, int* zseg) {
+  if (!c || nsteps < 1) return RTM_E_ARG;
+  if (!c->model_ok || !c->wav_ok || !c->shots_ok)
+    return fail(c, RTM_E_STATE, "model, wavelet and shots must be set");
+  static const int kCand[] = {8, 16, 24, 32, 48, 64, 96};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return fail(c, RTM_E_HIP, "hipEventCreate failed");
+  }
+  const int saved_timing = c->timing;
+  c->timing = 0;
+  int best = c->zs, rc = RTM_OK;
+  float best_ms = 3.0e38f;
+  const int steps = std::min(nsteps, c->p.nt - 3);
+  for (int z : kCand) {
+    c->zs = z;
+    rc = rtm_forward_begin(c, 0);
+    if (rc == RTM_OK) rc = forward_step(c, 2);  // first launch of this shape: not timed
+    if (rc != RTM_OK) break;
+    hipError_t e = hipEventRecord(e0, c->stream);
+    for (int i = 0; i < steps && rc == RTM_OK; ++i) rc = forward_step(c, 3 + i);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) rc = fail(c, RTM_E_HIP, "autotune timing: %s", hipGetErrorString(e));
+    if (rc != RTM_OK) break;
+    if (ms < best_ms) { best_ms = ms; best = z; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->timing = saved_timing;
+  c->zs = best;
+  c->p.zseg = best;  // kept across later batches of this context
+  c->kf = -1;
+  c->kb = -2;
+  drop_graph(c);
+  if (zseg) *zseg = best;
+  return rc;
+}
+
 int rtm_sync(rtm_ctx* c) {
   if (!c) return RTM_E_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -77,13 +77,14 @@ def lib():
         L.rtm_image_post.argtypes = [vp, C.c_float, C.c_float, vp, vp]
         L.rtm_stack_finish.argtypes = [vp, C.c_int, C.c_int, _fp, vp]
         L.rtm_set_timing.argtypes = [vp, C.c_int]
+        L.rtm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
                   "rtm_set_shot_geometry", "rtm_set_shot_model",
                   "rtm_set_gathers", "rtm_gathers_from_records", "rtm_forward_begin", "rtm_forward_advance",
                   "rtm_backward_begin", "rtm_backward_advance", "rtm_migrate", "rtm_set_graph", "rtm_sync",
                   "rtm_read_field", "rtm_read_records", "rtm_read_strip", "rtm_image_post",
-                  "rtm_stack_finish", "rtm_set_timing", "rtm_kernel_time"):
+                  "rtm_stack_finish", "rtm_set_timing", "rtm_kernel_time", "rtm_autotune"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
@@ -212,6 +213,12 @@ class Context:
         self._chk(lib().rtm_stack_finish(self.h, int(nrec), int(inorm), img, ill.ctypes.data),
                   "rtm_stack_finish")
         return img.reshape(mnz, self.gw), ill.reshape(mnz, self.gw)
+
+    def autotune(self, nsteps=4):
+        """Pick the interior row-segment height by timing forward steps; returns it."""
+        z = C.c_int()
+        self._chk(lib().rtm_autotune(self.h, int(nsteps), C.byref(z)), "rtm_autotune")
+        return z.value
 
     def set_timing(self, mask):
         """mask: bit set of kernel kinds (1 << K_*), 0 off, -1 all."""

@@ -255,3 +255,22 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(g.field(hip.FIELD_I2), ref_i2), f"replay {rep}"
         eager.migrate()  # unrelated eager work between replays
         eager.sync()
+
+
+def test_autotune_keeps_results():
+    """The autotuner only changes the row-segment height of the interior
+    kernels; images after tuning equal the default context's bit for bit."""
+    from rtm_gpu_amd import hip
+    case = build_case(name="batch3", nshot=3, depth_step=40.0)
+    seis = np.stack([synth_gather(case, s) for s in range(3)])
+    ref = hip_context(case, nshot=3)
+    ref.set_gathers(seis)
+    ref.migrate()
+    ctx = hip_context(case, nshot=3)
+    z = ctx.autotune(3)
+    assert z in (8, 16, 24, 32, 48, 64, 96)
+    ctx.set_gathers(seis)
+    ctx.migrate()
+    for s in range(3):
+        for which in (hip.FIELD_I1, hip.FIELD_I2):
+            assert np.array_equal(ctx.field(which, shot=s), ref.field(which, shot=s))
