@@ -115,6 +115,13 @@ struct Engine {
   int q0 = 0, q1 = 1, q2 = 2;
   std::vector<R> B;     // one-way scratch (DFWb)
   std::vector<R> SU, SQ, I1, I2;
+  // linearised (Born) modelling and its adjoint (not in the reference)
+  std::vector<R> D[3];   // scattered field
+  int d0 = 0, d1 = 1, d2 = 2;
+  std::vector<float> refl;  // reflectivity m, full grid (used on the interior)
+  std::vector<R> G;         // sum_k D2u(k+1) * mu(k+2)
+  mutable std::vector<R> Gimg;  // G / P (refreshed by field 9 reads)
+  bool born = false;
   std::vector<R> strips;
   std::vector<int> snap_slot;  // per step k: snapshot slot or -1
   std::vector<R> snaps;
@@ -323,7 +330,7 @@ struct Engine {
   }
 
   void record_step(int k, const R* U) {
-    if (!g.record) return;
+    if (!g.record || born) return;
     for (int s = 0; s < g.n; ++s)
       rec[(size_t)s * NT + k] = (float)U[(size_t)g.s_z * NX + g.s_l + s * g.ds];
   }
@@ -470,8 +477,159 @@ struct Engine {
     return kb;
   }
 
+  // ---- Born modelling: the scattered field dU follows the background step
+  // with the same update and ABC, driven on the interior by m times the
+  // background's second time difference:
+  //   dU(k) = update(dU(k-1), dU(k-2)) + m * ((u(k) - 2u(k-1)) + u(k-2)).
+  // Its receiver record is the Born data.
+  int born_begin() {
+    for (int i = 0; i < 3; ++i) D[i].assign(NXZ, R(0));
+    d0 = 0; d1 = 1; d2 = 2;
+    born = false;
+    int k = forward_begin();
+    born = true;
+    if (g.record) std::fill(rec.begin(), rec.end(), 0.f);
+    return k;
+  }
+  struct BornArgs { Engine* e; };
+  static void born_rows(void* a_, int b, int e) {
+    Engine& E = *((BornArgs*)a_)->e;
+    bool f64 = (E.g.ilste != 0);
+    const R* U1 = E.D[E.d1].data();
+    const R* U0 = E.D[E.d0].data();
+    R* U2 = E.D[E.d2].data();
+    const R* uk = E.F[E.f1].data();   // after the background rotation: u(k)
+    const R* uk1 = E.F[E.f0].data();  // u(k-1)
+    const R* uk2 = E.F[E.f2].data();  // u(k-2)
+    for (int x = b; x < e; ++x)
+      for (int y = 0; y < E.NX; ++y) {
+        size_t p = (size_t)x * E.NX + y;
+        R w1 = E.w1_at(U1, x, y);
+        R o = E.combine(U1[p], U0[p], E.v[p], w1, f64);
+        if (x >= E.N2 && x < E.NZ - E.N2 && y >= E.N2 && y < E.NX - E.N2) {
+          R d2u = (uk[p] - (R)2 * uk1[p]) + uk2[p];
+          o = fmaR<R>((R)E.refl[p], d2u, o);
+        }
+        U2[p] = o;
+      }
+  }
+  void born_step(int k) {
+    forward_step(k);  // background u(k); rotation leaves u(k), u(k-1), u(k-2) in f1, f0, f2
+    BornArgs a{this};
+    pool.run(NZ, born_rows, &a);
+    hybrid1(B.data(), D[d2].data(), D[d1].data(), D[d0].data());
+    hybrid2(D[d2].data(), B.data());
+    if (g.record)
+      for (int s = 0; s < g.n; ++s)
+        rec[(size_t)s * NT + k] = (float)D[d2][(size_t)g.s_z * NX + g.s_l + s * g.ds];
+    int t = d0; d0 = d1; d1 = d2; d2 = t;
+  }
+  int born_advance(int nsteps) {
+    for (int i = 0; i < nsteps && kf < NT; ++i, ++kf) born_step(kf);
+    fwd_done = (kf >= NT);
+    return kf;
+  }
+
+  // ---- adjoint of Born modelling.  The update is A = 2I + P*L with
+  // P = v^2*tao2*h2 per cell, so A^T = P^-1 A P: the adjoint recursion
+  // lambda(k) = A^T lambda(k+1) - lambda(k+2) + R^T r(k) is carried as
+  // mu = P*lambda, which takes the same update A and ABC with P*r(k) added at
+  // the receivers (exact when L is symmetric, i.e. one coefficient row for all
+  // cells).  The background is reconstructed backward from the strips as in
+  // the migration.  G accumulates ((u(k+2) - 2u(k+1)) + u(k)) * mu(k+2) on
+  // the interior; the gradient is G / P (adjoint_image).  mu(nt-1) and
+  // mu(nt-2) come from two updates before the loop; their image terms pair
+  // with mu(nt), mu(nt+1) = 0.
+  R pcell(size_t p) const { return (R)(v[p] * v[p] * g.tao2 * g.h2); }
+  void adj_inject(int k) {
+    for (int s = 0; s < g.n; ++s) {
+      size_t p = (size_t)g.s_z * NX + g.s_l + s * g.ds;
+      Q[q2][p] = Q[q2][p] + pcell(p) * (R)seis[(size_t)s * NT + k];
+    }
+  }
+  void adj_update(int k) {
+    AdjArgs a{this, k};
+    pool.run(NZ, adj_rows, &a);
+    adj_inject(k);
+    hybrid1(B.data(), Q[q2].data(), Q[q1].data(), Q[q0].data());
+    hybrid2(Q[q2].data(), B.data());
+  }
+  int adjoint_begin(const float* resid) {
+    born = false;
+    backward_begin(resid);  // F roles, strips; Q, images set below
+    for (int i = 0; i < 3; ++i) std::fill(Q[i].begin(), Q[i].end(), R(0));
+    q0 = 0; q1 = 1; q2 = 2;
+    G.assign(NXZ, R(0));
+    for (int k = NT - 1; k >= NT - 2; --k) {  // mu(nt-1), mu(nt-2)
+      adj_update(k);
+      int t = q0; q0 = q1; q1 = q2; q2 = t;
+    }
+    return kb;
+  }
+  void adjoint_image(R* out) const {  // G / P on the interior, 0 elsewhere
+    for (size_t p = 0; p < NXZ; ++p) out[p] = R(0);
+    for (int x = N2; x < NZ - N2; ++x)
+      for (int y = N2; y < NX - N2; ++y) {
+        size_t p = (size_t)x * NX + y;
+        out[p] = G[p] / pcell(p);
+      }
+  }
+  struct AdjArgs { Engine* e; int k; };
+  static void adj_rows(void* a_, int b, int e) {
+    AdjArgs* a = (AdjArgs*)a_;
+    Engine& E = *a->e;
+    const R* Q1 = E.Q[E.q1].data();
+    const R* Q0 = E.Q[E.q0].data();
+    R* Q2 = E.Q[E.q2].data();
+    for (int x = b; x < e; ++x)
+      for (int y = 0; y < E.NX; ++y) {
+        size_t p = (size_t)x * E.NX + y;
+        R w1 = E.w1_at(Q1, x, y);
+        Q2[p] = E.combine(Q1[p], Q0[p], E.v[p], w1, false);
+      }
+  }
+  static void adj_image_rows(void* a_, int b, int e) {
+    Engine& E = *((AdjArgs*)a_)->e;
+    const R* u0 = E.F[E.f1].data();  // u(k)
+    const R* u1 = E.F[E.f0].data();  // u(k+1)
+    const R* u2 = E.F[E.f2].data();  // u(k+2)
+    const R* lam = E.Q[E.q0].data(); // lambda(k+2), before the Q rotation
+    for (int x = b; x < e; ++x) {
+      if (x < E.N2 || x >= E.NZ - E.N2) continue;
+      for (int y = E.N2; y < E.NX - E.N2; ++y) {
+        size_t p = (size_t)x * E.NX + y;
+        R d2u = (u2[p] - (R)2 * u1[p]) + u0[p];
+        E.G[p] = fmaR<R>(d2u, lam[p], E.G[p]);
+      }
+    }
+  }
+  void adjoint_step(int k) {
+    R wv = (k < NT2) ? (R)wav[k + 1] : R(0);  // background reconstruction as in the migration
+    restore_strips(k + 1, F[f1].data());
+    BkArgs ab{this, wv};
+    pool.run(NZ, bk_src_rows, &ab);
+    { int t = f0; f0 = f1; f1 = f2; f2 = t; }
+    if (snap_slot[k] >= 0)
+      std::copy(snaps.begin() + NXZ * snap_slot[k], snaps.begin() + NXZ * (snap_slot[k] + 1), F[f1].begin());
+    adj_update(k);
+    AdjArgs a{this, k};
+    pool.run(NZ, adj_image_rows, &a);
+    { int t = q0; q0 = q1; q1 = q2; q2 = t; }
+  }
+  int adjoint_advance(int nsteps) {
+    for (int i = 0; i < nsteps && kb >= 0; ++i, --kb) adjoint_step(kb);
+    return kb;
+  }
+
   const R* field_ptr(int which) const {
+    if (which == 9) {
+      Gimg.resize(NXZ);
+      adjoint_image(Gimg.data());
+    }
     switch (which) {
+      case 7: return D[d1].data();
+      case 8: return G.data();
+      case 9: return Gimg.data();
       case 0: return F[f1].data();
       case 1: return F[f0].data();
       case 2: return Q[q1].data();
@@ -919,6 +1077,14 @@ int orc_forward_begin(orc_engine* o) { return ORC_DISPATCH(o, forward_begin()); 
 int orc_forward_advance(orc_engine* o, int n) { return ORC_DISPATCH(o, forward_advance(n)); }
 int orc_backward_begin(orc_engine* o, const float* seis) { return ORC_DISPATCH(o, backward_begin(seis)); }
 int orc_backward_advance(orc_engine* o, int n) { return ORC_DISPATCH(o, backward_advance(n)); }
+void orc_set_reflectivity(orc_engine* o, const float* m) {
+  if (o->e64) o->e64->refl.assign(m, m + o->e64->NXZ);
+  else o->e32->refl.assign(m, m + o->e32->NXZ);
+}
+int orc_born_begin(orc_engine* o) { return ORC_DISPATCH(o, born_begin()); }
+int orc_born_advance(orc_engine* o, int n) { return ORC_DISPATCH(o, born_advance(n)); }
+int orc_adjoint_begin(orc_engine* o, const float* resid) { return ORC_DISPATCH(o, adjoint_begin(resid)); }
+int orc_adjoint_advance(orc_engine* o, int n) { return ORC_DISPATCH(o, adjoint_advance(n)); }
 void orc_read_field(const orc_engine* o, int which, float* out) {
   if (o->e64) o->e64->read_field<float>(which, out);
   else o->e32->read_field<float>(which, out);

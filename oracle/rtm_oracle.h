@@ -86,9 +86,28 @@ int orc_forward_advance(orc_engine* e, int nsteps);
 /* Backward: kernel.cu:822-931.  seis [n][nt] (trace-major, reference layout). */
 int orc_backward_begin(orc_engine* e, const float* seis);
 int orc_backward_advance(orc_engine* e, int nsteps);
+/* Linearised (Born) modelling and its adjoint (not in the reference).
+ * m: reflectivity [nz*nx] (used on the interior).  born: the background
+ * forward step, then the scattered field dU(k) = update(dU(k-1), dU(k-2)) +
+ * m*((u(k)-2u(k-1))+u(k-2)) with the same ABC; with record = 1 the record
+ * holds dU at the receivers.  adjoint (after a plain forward pass with
+ * strips): the background is reconstructed backward from the strips (and
+ * snapshots) as in the migration; the adjoint field is carried as
+ * mu = P*lambda, P = v^2*tao2*h2 per cell, so it takes the plain update
+ * (FP32 combine) and ABC with P*resid[s][k] added at the receivers after the
+ * update; mu(nt-1), mu(nt-2) come from two updates in adjoint_begin, and
+ * G += ((u(k+2)-2u(k+1))+u(k)) * mu(k+2) on the interior.  The gradient is
+ * G / P (field 9). */
+void orc_set_reflectivity(orc_engine* e, const float* m);
+int orc_born_begin(orc_engine* e);
+int orc_born_advance(orc_engine* e, int nsteps);
+int orc_adjoint_begin(orc_engine* e, const float* resid);
+int orc_adjoint_advance(orc_engine* e, int nsteps);
 /* which: 0 = current forward/source field u (u(k) most recent),
  *        1 = previous forward/source field,
- *        2 = receiver field q (most recent), 3 = I1, 4 = I2, 5 = sum U, 6 = sum Q.
+ *        2 = receiver field q (most recent), 3 = I1, 4 = I2, 5 = sum U, 6 = sum Q,
+ *        7 = scattered field dU (most recent), 8 = adjoint accumulator G,
+ *        9 = gradient G / P (interior; 0 on the frame).
  * Output is float [nz*nx] (images: interior valid, frame 0). */
 void orc_read_field(const orc_engine* e, int which, float* out);
 void orc_read_field64(const orc_engine* e, int which, double* out);

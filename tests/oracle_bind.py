@@ -67,14 +67,17 @@ def lib():
         L.orc_create.restype = C.c_void_p
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_set_model.argtypes = [C.c_void_p, fp, fp, fp, fp, C.c_int, C.c_void_p, C.c_int, fp]
-        for n in ("orc_forward_begin",):
+        for n in ("orc_forward_begin", "orc_born_begin"):
             getattr(L, n).argtypes = [C.c_void_p]
             getattr(L, n).restype = C.c_int
-        for n in ("orc_forward_advance", "orc_backward_advance"):
+        for n in ("orc_forward_advance", "orc_backward_advance", "orc_born_advance",
+                  "orc_adjoint_advance"):
             getattr(L, n).argtypes = [C.c_void_p, C.c_int]
             getattr(L, n).restype = C.c_int
-        L.orc_backward_begin.argtypes = [C.c_void_p, fp]
-        L.orc_backward_begin.restype = C.c_int
+        for n in ("orc_backward_begin", "orc_adjoint_begin"):
+            getattr(L, n).argtypes = [C.c_void_p, fp]
+            getattr(L, n).restype = C.c_int
+        L.orc_set_reflectivity.argtypes = [C.c_void_p, fp]
         L.orc_read_field.argtypes = [C.c_void_p, C.c_int, fp]
         L.orc_read_field64.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_read_record.argtypes = [C.c_void_p, fp]
@@ -237,6 +240,23 @@ class Engine:
 
     def backward_advance(self, n):
         return lib().orc_backward_advance(self.h, n)
+
+    def set_reflectivity(self, m):
+        self._refl = np.ascontiguousarray(m, np.float32).ravel()
+        lib().orc_set_reflectivity(self.h, self._refl)
+
+    def born_begin(self):
+        return lib().orc_born_begin(self.h)
+
+    def born_advance(self, n):
+        return lib().orc_born_advance(self.h, n)
+
+    def adjoint_begin(self, resid):
+        self._seis = np.ascontiguousarray(resid, np.float32).ravel()
+        return lib().orc_adjoint_begin(self.h, self._seis)
+
+    def adjoint_advance(self, n):
+        return lib().orc_adjoint_advance(self.h, n)
 
     def field(self, which, f64=False):
         if f64:
