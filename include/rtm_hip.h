@@ -46,12 +46,15 @@ enum {
   RTM_FIELD_I2 = 4,     /* source illumination (Drel2) */
   RTM_FIELD_SU = 5,     /* running sum of u (DBWADD) */
   RTM_FIELD_SQ = 6      /* running sum of q (DFWADD) */
+  /* Born pass: RTM_FIELD_Q holds the scattered field dU; adjoint pass:
+   * RTM_FIELD_Q holds mu = P*lambda and RTM_FIELD_I1 the accumulator G. */
 };
 
 /* Kernel kinds for rtm_kernel_time(). */
 enum {
   RTM_K_FWD_INTERIOR = 0, RTM_K_FWD_FRAME = 1, RTM_K_SRC_INTERIOR = 2, RTM_K_STRIP_RESTORE = 3,
-  RTM_K_RCV_INTERIOR = 4, RTM_K_RCV_FRAME = 5, RTM_K_NKINDS = 6
+  RTM_K_RCV_INTERIOR = 4, RTM_K_RCV_FRAME = 5, RTM_K_BORN_INTERIOR = 6, RTM_K_BORN_FRAME = 7,
+  RTM_K_ADJ_INTERIOR = 8, RTM_K_ADJ_FRAME = 9, RTM_K_NKINDS = 10
 };
 
 typedef struct rtm_params {
@@ -165,6 +168,40 @@ int rtm_image_post(rtm_ctx* ctx, float vmax, float whitecoe, float* up, float* d
  * img /= illum when inorm = 1.  [mnz][stack_nx] (stack_nx = mnx unless set).
  * Resets the stack. */
 int rtm_stack_finish(rtm_ctx* ctx, int nrec, int inorm, float* img, float* illum);
+
+/* Least-squares migration operators (not in the reference, which migrates
+ * only).  L is Born (linearised) modelling: the scattered field dU follows
+ * the background forward step with the same update and hybrid ABC, driven on
+ * the interior by the reflectivity m:
+ *   dU(k) = update(dU(k-1), dU(k-2)) + m*((u(k) - 2u(k-1)) + u(k-2)),
+ * recorded at the receivers.  The adjoint L^T back-propagates residuals r
+ * with the background reconstructed from the strips (and snapshots) as in
+ * the migration; the adjoint field is carried as mu = P*lambda with
+ * P = v^2*tao2*h2 per cell, so it takes the plain update and ABC with
+ * P*r(k) added at the receivers, and the gradient is
+ * (sum_k ((u(k+2) - 2u(k+1)) + u(k)) * mu(k+2)) / P.  The pair passes the
+ * dot-product test to the accuracy of the ABC when the model has one
+ * coefficient row (ilste = 1); per-cell LS rows make the spatial operator
+ * non-symmetric. */
+
+/* Reflectivity m on the stacked image grid [mnz][stack_nx]; each shot reads
+ * its window at x_img.  Needs the shots to be set. */
+int rtm_set_reflectivity(rtm_ctx* ctx, const float* m);
+/* Born pass of the batch: begin starts the background (with strips, so that
+ * rtm_adjoint_begin may follow) and zeroes dU and the records; advance runs
+ * up to nsteps of k = 2..nt-1.  Needs a context with record = 1; the Born
+ * data are read with rtm_read_records. */
+int rtm_born_begin(rtm_ctx* ctx);
+int rtm_born_advance(rtm_ctx* ctx, int nsteps);
+/* Adjoint pass after a complete forward or Born pass with strips, with the
+ * residuals r[shot][n][nt] given by rtm_set_gathers: begin zeroes mu and the
+ * gradient accumulator and takes mu(nt-1), mu(nt-2) (two updates); advance
+ * runs up to nsteps of k = nt-3..0. */
+int rtm_adjoint_begin(rtm_ctx* ctx);
+int rtm_adjoint_advance(rtm_ctx* ctx, int nsteps);
+/* Gradient of the batch after a complete adjoint pass: grad[mnz][stack_nx] =
+ * sum over shots, in shot order, of each shot's G/P at its image window. */
+int rtm_adjoint_gradient(rtm_ctx* ctx, float* grad);
 
 /* Autotuner (not in the reference): times nsteps forward steps of the
  * current batch for each candidate height of the interior kernels' row

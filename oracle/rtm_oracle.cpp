@@ -643,7 +643,7 @@ struct Engine {
   template <typename T>
   void read_field(int which, T* out) const {
     const R* f = field_ptr(which);
-    bool img = which >= 3;
+    bool img = which >= 3 && which != 7;  // 7: the scattered wavefield, frame included
     for (int z = 0; z < NZ; ++z)
       for (int x = 0; x < NX; ++x) {
         size_t p = (size_t)z * NX + x;

@@ -69,6 +69,14 @@ struct rtm_ctx {
   int nshot = 0;
   int f0 = 0, f1 = 1, f2 = 2, q0 = 0, q1 = 1, q2 = 2;
   int kf = -1, kb = -2;  // next forward / backward step; -1/-2 = not begun
+  // least-squares migration operators
+  bool born = false;     // the forward pass is a Born pass (records hold dU)
+  bool adj = false;      // the backward pass is the adjoint of Born modelling
+  float* d_refl_img = nullptr;  // reflectivity on the stacked image grid [mnz][gw]
+  float* d_refl = nullptr;      // padded reflectivity fields
+  size_t refl_cap = 0;
+  bool refl_ok = false;
+  float* d_grad = nullptr;      // [mnz][gw]
   bool model_ok = false, wav_ok = false, shots_ok = false, data_ok = false, save_strips = true;
   int zs = 32, nxt = 1;
   // captured migration (hipGraph)
@@ -190,6 +198,8 @@ void fill_int_args(rtm_ctx* c, IntArgs& a) {
   a.uk = nullptr;
   a.su = c->SU; a.sq = c->SQ; a.i1 = c->I1; a.i2 = c->I2;
   a.icompen = c->p.icompen;
+  a.ua = a.ub = a.uc = nullptr;
+  a.image = 0;
 }
 void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
   a.g = c->g;
@@ -202,6 +212,7 @@ void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
   a.data = c->d_data;
   a.st = c->st;
   a.save_strips = 0;
+  a.ua = a.ub = a.uc = nullptr;
 }
 
 int forward_step(rtm_ctx* c, int k) {
@@ -229,7 +240,7 @@ int forward_step(rtm_ctx* c, int k) {
   if (quant && k == c->p.nt - 1) HIPCHK(c, launch_strip_finish(c->g, c->st, c->d_geo, k, c->nshot, c->stream));
   if (c->save_strips && c->snap_slot[k] >= 0)  // checkpoint snapshot of u(k)
     HIPCHK(c, launch_snap_save(c->g, c->F[c->f2], c->sn, c->d_geo, c->snap_slot[k], c->nshot, c->stream));
-  if (c->p.record)  // receiver recording of u(k)
+  if (c->p.record && !c->born)  // receiver recording of u(k)
     HIPCHK(c, launch_record(c->g, c->r, c->d_geo, c->F[c->f2], c->d_rec, k, c->p.nt, c->nshot, c->stream));
   int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t;  // Deliver (kernel.cu:820)
   return RTM_OK;
@@ -271,6 +282,83 @@ int backward_step(rtm_ctx* c, int k) {
   }
   { int t = c->q0; c->q0 = c->q1; c->q1 = c->q2; c->q2 = t; }  // Deliver (kernel.cu:921)
   return RTM_OK;
+}
+
+// Born step k: the background step, then the scattered field from dU(k-1),
+// dU(k-2) and D2u = (u(k) - 2u(k-1)) + u(k-2) (the background rotation
+// leaves u(k), u(k-1), u(k-2) in f1, f0, f2), recorded at the receivers.
+int born_step(rtm_ctx* c, int k) {
+  int rc = forward_step(c, k);
+  if (rc != RTM_OK) return rc;
+  {
+    IntArgs a;
+    fill_int_args(c, a);
+    a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
+    a.ua = c->F[c->f1]; a.ub = c->F[c->f0]; a.uc = c->F[c->f2];
+    a.wav = 0.f; a.k = k;
+    Timed t(c, RTM_K_BORN_INTERIOR);
+    HIPCHK(c, launch_interior(c->p.ilste ? EPI_BORN64 : EPI_BORN, c->nfd, a, c->nshot, c->stream));
+  }
+  {
+    FrameArgs a;
+    fill_frame_args(c, a);
+    a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
+    a.ua = c->F[c->f1]; a.ub = c->F[c->f0]; a.uc = c->F[c->f2];
+    a.wav = 0.f; a.k = k;
+    Timed t(c, RTM_K_BORN_FRAME);
+    HIPCHK(c, launch_frame(FR_BORN, c->nfd, a, c->nshot, c->ntiles, c->stream));
+  }
+  HIPCHK(c, launch_record(c->g, c->r, c->d_geo, c->Q[c->q2], c->d_rec, k, c->p.nt, c->nshot, c->stream));
+  int t = c->q0; c->q0 = c->q1; c->q1 = c->q2; c->q2 = t;
+  return RTM_OK;
+}
+
+// One update of the adjoint field: mu(k) from mu(k+1), mu(k+2) with P*r(k)
+// at the receivers; with image, G += D2u * mu(k+2), D2u = (u(k+2) - 2u(k+1))
+// + u(k) (the background rotation leaves u(k), u(k+1), u(k+2) in f1, f0, f2).
+int adjoint_update(rtm_ctx* c, int k, bool image) {
+  {
+    IntArgs a;
+    fill_int_args(c, a);
+    a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
+    a.ua = c->F[c->f2]; a.ub = c->F[c->f0]; a.uc = c->F[c->f1];
+    a.image = image ? 1 : 0;
+    a.wav = 0.f; a.k = k;
+    Timed t(c, RTM_K_ADJ_INTERIOR);
+    HIPCHK(c, launch_interior(EPI_ADJ, c->nfd, a, c->nshot, c->stream));
+  }
+  {
+    FrameArgs a;
+    fill_frame_args(c, a);
+    a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
+    a.wav = 0.f; a.k = k;
+    Timed t(c, RTM_K_ADJ_FRAME);
+    HIPCHK(c, launch_frame(FR_ADJ, c->nfd, a, c->nshot, c->ntiles, c->stream));
+  }
+  int t = c->q0; c->q0 = c->q1; c->q1 = c->q2; c->q2 = t;
+  return RTM_OK;
+}
+
+// Adjoint step k: the background u(k) reconstructed as in backward_step,
+// then the adjoint update with imaging.
+int adjoint_step(rtm_ctx* c, int k) {
+  const float wv = (k < c->p.nt2) ? c->wav[k + 1] : 0.0f;
+  {
+    IntArgs a;
+    fill_int_args(c, a);
+    a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
+    a.wav = wv; a.k = k;
+    Timed t(c, RTM_K_SRC_INTERIOR);
+    HIPCHK(c, launch_interior(EPI_SRC, c->nfd, a, c->nshot, c->stream));
+  }
+  {
+    Timed t(c, RTM_K_STRIP_RESTORE);
+    HIPCHK(c, launch_strip_restore(c->g, c->F[c->f2], c->st, c->d_geo, k, c->nshot, c->stream));
+  }
+  if (c->snap_slot[k] >= 0)
+    HIPCHK(c, launch_snap_restore(c->g, c->F[c->f2], c->sn, c->d_geo, c->snap_slot[k], c->nshot, c->stream));
+  { int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t; }
+  return adjoint_update(c, k, true);
 }
 
 // Frame tiles of a grid nx columns wide, tagged with the shot.  Shapes must
@@ -684,6 +772,7 @@ int rtm_forward_begin(rtm_ctx* c, int save_strips) {
       if (!c->shot_model[s]) return fail(c, RTM_E_STATE, "shot %d has no model window", s);
   c->f0 = 0; c->f1 = 1; c->f2 = 2;
   c->save_strips = save_strips != 0;
+  c->born = false;
   const float spike = (float)((double)c->wav[0] / 2.0);  // f(0.0,f0)/2.0 (kernel.cu:803)
   HIPCHK(c, launch_fwd_begin(c->g, c->F[0], c->F[1], c->d_geo, spike, c->nshot, c->stream));
   if (c->save_strips) {  // Equal (kernel.cu:808)
@@ -723,6 +812,7 @@ int rtm_backward_begin(rtm_ctx* c) {
   // DBW1 = u(NT-2), DBW0 = u(NT-1) (kernel.cu:822-825): swap roles of f0/f1.
   std::swap(c->f0, c->f1);
   c->q0 = 0; c->q1 = 1; c->q2 = 2;
+  c->adj = false;
   const float spike = (float)((double)c->wav[0] / 2.0);
   HIPCHK(c, launch_bwd_begin(c->g, c->F[c->f0], c->F[c->f1], c->Q[0], c->Q[1], c->SU, c->SQ,
                              c->I1, c->I2, c->d_geo, spike, c->p.icompen, c->nshot, c->stream));
@@ -739,6 +829,108 @@ int rtm_backward_advance(rtm_ctx* c, int nsteps) {
     int rc = backward_step(c, c->kb);
     if (rc != RTM_OK) return rc;
   }
+  return RTM_OK;
+}
+
+int rtm_set_reflectivity(rtm_ctx* c, const float* m) {
+  if (!c || !m) return RTM_E_ARG;
+  const size_t n = (size_t)c->g.mnz * c->gw;
+  if (!c->d_refl_img) {
+    int rc = dalloc_t(c, &c->d_refl_img, n);
+    if (rc != RTM_OK) return rc;
+  }
+  HIPCHK(c, hipMemcpy(c->d_refl_img, m, n * sizeof(float), hipMemcpyHostToDevice));
+  c->refl_ok = true;
+  return RTM_OK;
+}
+
+int rtm_born_begin(rtm_ctx* c) {
+  if (!c) return RTM_E_ARG;
+  if (!c->p.record || !c->d_rec) return fail(c, RTM_E_STATE, "Born modelling needs a context with record = 1");
+  if (!c->refl_ok) return fail(c, RTM_E_STATE, "rtm_set_reflectivity first");
+  for (int s = 0; s < c->nshot; ++s)
+    if (c->geo[s].x_img + c->geo[s].mnx > c->gw)
+      return fail(c, RTM_E_ARG, "shot %d image columns %d..%d outside the image width %d", s,
+                  c->geo[s].x_img, c->geo[s].x_img + c->geo[s].mnx, c->gw);
+  int rc = rtm_forward_begin(c, 1);
+  if (rc != RTM_OK) return rc;
+  // each shot reads its window of the reflectivity; a uniform batch shares one field
+  const Grid& g = c->g;
+  bool uniform = true;
+  for (int s = 0; s < c->nshot; ++s) uniform = uniform && c->geo[s].nx == g.nx && c->geo[s].x_img == 0;
+  const long long rstride = uniform ? 0 : g.fstride;
+  const size_t need = (size_t)(uniform ? 1 : c->nshot) * g.fstride;
+  if (c->refl_cap < need) {
+    const size_t cap = uniform ? need : (size_t)c->p.max_shots * g.fstride;
+    rc = dalloc_t(c, &c->d_refl, cap);
+    if (rc != RTM_OK) return rc;
+    c->refl_cap = cap;
+  }
+  HIPCHK(c, launch_refl_scatter(g, c->d_geo, c->gw, c->d_refl_img, c->d_refl, rstride, c->nshot, c->stream));
+  c->m.refl = c->d_refl;
+  c->m.rstride = rstride;
+  const long long fb = (long long)c->nshot * g.fstride;
+  for (int i = 0; i < 3; ++i) HIPCHK(c, launch_fill(c->Q[i], fb, 0.f, c->stream));
+  HIPCHK(c, launch_fill(c->d_rec, (long long)c->nshot * c->p.nt * c->p.n, 0.f, c->stream));
+  c->q0 = 0; c->q1 = 1; c->q2 = 2;
+  c->born = true;
+  return RTM_OK;
+}
+
+int rtm_born_advance(rtm_ctx* c, int nsteps) {
+  if (!c) return RTM_E_ARG;
+  if (!c->born || c->kf < 2) return fail(c, RTM_E_STATE, "rtm_born_begin first");
+  for (int i = 0; i < nsteps && c->kf < c->p.nt; ++i, ++c->kf) {
+    int rc = born_step(c, c->kf);
+    if (rc != RTM_OK) return rc;
+  }
+  return RTM_OK;
+}
+
+int rtm_adjoint_begin(rtm_ctx* c) {
+  if (!c) return RTM_E_ARG;
+  if (c->kf != c->p.nt) return fail(c, RTM_E_STATE, "forward pass incomplete (k=%d of %d)", c->kf, c->p.nt);
+  if (!c->save_strips) return fail(c, RTM_E_STATE, "forward pass ran without strips");
+  if (!c->data_ok) return fail(c, RTM_E_STATE, "rtm_set_gathers first (residuals)");
+  std::swap(c->f0, c->f1);  // f1 = u(nt-2), f0 = u(nt-1), as for the migration
+  const long long fb = (long long)c->nshot * c->g.fstride;
+  for (int i = 0; i < 3; ++i) HIPCHK(c, launch_fill(c->Q[i], fb, 0.f, c->stream));
+  HIPCHK(c, launch_fill(c->I1, fb, 0.f, c->stream));
+  HIPCHK(c, launch_strip_restore(c->g, c->F[c->f1], c->st, c->d_geo, c->p.nt - 2, c->nshot, c->stream));
+  c->q0 = 0; c->q1 = 1; c->q2 = 2;
+  c->adj = true;
+  c->kb = -2;
+  c->kf = -1;  // the reconstruction consumes the forward end state
+  for (int k = c->p.nt - 1; k >= c->p.nt - 2; --k) {  // mu(nt-1), mu(nt-2)
+    int rc = adjoint_update(c, k, false);
+    if (rc != RTM_OK) return rc;
+  }
+  c->kb = c->p.nt - 3;
+  return RTM_OK;
+}
+
+int rtm_adjoint_advance(rtm_ctx* c, int nsteps) {
+  if (!c) return RTM_E_ARG;
+  if (!c->adj || c->kb < -1) return fail(c, RTM_E_STATE, "rtm_adjoint_begin first");
+  for (int i = 0; i < nsteps && c->kb >= 0; ++i, --c->kb) {
+    int rc = adjoint_step(c, c->kb);
+    if (rc != RTM_OK) return rc;
+  }
+  return RTM_OK;
+}
+
+int rtm_adjoint_gradient(rtm_ctx* c, float* grad) {
+  if (!c || !grad) return RTM_E_ARG;
+  if (!c->adj || c->kb != -1) return fail(c, RTM_E_STATE, "adjoint pass incomplete");
+  const size_t n = (size_t)c->g.mnz * c->gw;
+  if (!c->d_grad) {
+    int rc = dalloc_t(c, &c->d_grad, n);
+    if (rc != RTM_OK) return rc;
+  }
+  HIPCHK(c, launch_grad_stack(c->g, c->d_geo, c->gw, c->I1, c->m.pm, c->m.mstride, c->nshot,
+                              c->d_grad, c->stream));
+  HIPCHK(c, hipMemcpyAsync(grad, c->d_grad, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return RTM_OK;
 }
 
@@ -917,7 +1109,7 @@ int rtm_read_strip(rtm_ctx* c, int shot, int k, float* out) {
 
 int rtm_image_post(rtm_ctx* c, float vmax, float whitecoe, float* up, float* down) {
   if (!c) return RTM_E_ARG;
-  if (c->kb != -1) return fail(c, RTM_E_STATE, "backward pass incomplete");
+  if (c->kb != -1 || c->adj) return fail(c, RTM_E_STATE, "backward pass incomplete");
   const Grid& g = c->g;
   const int nI = g.mnz * g.mnx;  // per shot: [mnz][mnx of the widest shot]
   HIPCHK(c, launch_fill(reinterpret_cast<float*>(c->d_imax), c->nshot, 0.f, c->stream));

@@ -26,6 +26,8 @@ struct Model {
   const unsigned* ko;  // byte offset of the cell's coefficient row in ctab, padded layout
   const float* ctab;
   const float* w;
+  const float* refl;   // reflectivity (Born modelling), padded layout, per shot at rstride
+  long long rstride;   // 0: one reflectivity field for all shots
   int cs, nrow;
   float hzx2_1, tao2, h2, taoh, taoh2;
 };
@@ -63,8 +65,12 @@ struct Snaps {
   int fmt, nsnap;
 };
 
-enum { EPI_FWD = 0, EPI_SRC = 1, EPI_RCV = 2, EPI_FWD64 = 3 };
-enum { FR_FWD = 0, FR_RCV = 1 };
+// Interior epilogues: forward (FP32 / FP64 combine), source reconstruction,
+// receiver step + imaging, Born scattered field (FP32 / FP64 combine), and
+// the adjoint of Born modelling.
+enum { EPI_FWD = 0, EPI_SRC = 1, EPI_RCV = 2, EPI_FWD64 = 3, EPI_BORN = 4, EPI_BORN64 = 5,
+       EPI_ADJ = 6 };
+enum { FR_FWD = 0, FR_RCV = 1, FR_BORN = 2, FR_ADJ = 3 };
 
 struct IntArgs {
   Grid g;
@@ -82,6 +88,11 @@ struct IntArgs {
   const float* uk;     // u(k) for imaging (receiver mode)
   float *su, *sq, *i1, *i2;
   int icompen;
+  // Born / adjoint: background fields ua, ub, uc with D2u = (ua - 2ub) + uc
+  // (Born: u(k), u(k-1), u(k-2); adjoint: u(k+2), u(k+1), u(k)); the adjoint
+  // accumulates into i1 when image != 0 and reads data at step k.
+  const float *ua, *ub, *uc;
+  int image;
 };
 
 struct FrameArgs {
@@ -100,6 +111,7 @@ struct FrameArgs {
   const float* data;
   Strips st;
   int save_strips;
+  const float *ua, *ub, *uc;  // Born background fields (as in IntArgs)
 };
 
 // Launchers (rtm_kernels.hip).  nfd_inst is the compile-time stencil
@@ -140,6 +152,14 @@ hipError_t launch_post(const Grid& g, const ShotGeo* geo, const float* v, long l
 // in shot order per cell.
 hipError_t launch_stack_add(const Grid& g, const ShotGeo* geo, int gw, const float* up,
                             const float* down, float* s1, float* s2, int nshot, hipStream_t s);
+// Reflectivity image [mnz][gw] into padded fields at dst + shot*rstride (each
+// shot's window at x_img; zero off the interior); rstride = 0: shot 0 only.
+hipError_t launch_refl_scatter(const Grid& g, const ShotGeo* geo, int gw, const float* img,
+                               float* dst, long long rstride, int nshot, hipStream_t s);
+// grad[mnz][gw] = sum over shots, in shot order, of G/pm at each shot's window.
+hipError_t launch_grad_stack(const Grid& g, const ShotGeo* geo, int gw, const float* G,
+                             const float* pm, long long mstride, int nshot, float* grad,
+                             hipStream_t s);
 hipError_t launch_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
                                float* img, float* illum, hipStream_t s);
 

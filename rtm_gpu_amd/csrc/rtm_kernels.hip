@@ -239,9 +239,12 @@ __device__ __forceinline__ float strip_get(const Strips& st, int shot, int k, lo
 // end of the last segment are computed from clamped rows and not stored.
 template <int EPI>
 struct EpiTraits {
-  static constexpr bool f64 = EPI == EPI_FWD64 || EPI == EPI_SRC;  // "2.0*" combine
+  static constexpr bool f64 = EPI == EPI_FWD64 || EPI == EPI_SRC || EPI == EPI_BORN64;  // "2.0*"
   static constexpr bool rcv = EPI == EPI_RCV;
-  static constexpr int PD = rcv ? 1 : 2;  // prefetch distance of the stencil operands
+  static constexpr bool born = EPI == EPI_BORN || EPI == EPI_BORN64;
+  static constexpr bool adj = EPI == EPI_ADJ;
+  static constexpr bool src = !rcv && !born && !adj;  // injects the source wavelet
+  static constexpr int PD = (rcv || born || adj) ? 1 : 2;  // prefetch distance of the stencil operands
 };
 
 template <int NFD, int EPI>
@@ -307,14 +310,19 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) srcc[jj] = c0 + jj == sp.y;
 
-  // receiver step: data replacement of row s_z (kernel.cu:349-354), read once
+  // receiver step: data replacement of row s_z (kernel.cu:349-354), read once;
+  // adjoint: the residual of step k added (times P) at the receivers
   float dval[4] = {0.f, 0.f, 0.f, 0.f};
-  if (T::rcv && a.r.s_z >= zs && a.r.s_z < min(ze, zhi)) {
+  bool rflag[4] = {false, false, false, false};
+  if ((T::rcv || T::adj) && a.r.s_z >= zs && a.r.s_z < min(ze, zhi)) {
+    const int kd = T::rcv ? a.k + 1 : a.k;
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
       int s;
-      if (colok[jj] && is_receiver(c0 + jj, rr, &s))
-        dval[jj] = a.data[((long long)shot * a.nt + (a.k + 1)) * a.r.n + s];
+      if (colok[jj] && is_receiver(c0 + jj, rr, &s)) {
+        dval[jj] = a.data[((long long)shot * a.nt + kd) * a.r.n + s];
+        rflag[jj] = true;
+      }
     }
   }
   Rsrc RU, RI1, RI2, RSU, RSQ;
@@ -324,6 +332,15 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
     RI1 = rsrc(a.i1 + so, fb); RI2 = rsrc(a.i2 + so, fb);
     RSU = rsrc(a.su + so, fb); RSQ = rsrc(a.sq + so, fb);
     if (a.icompen) { vcs = vcol; vss = vst; }
+  }
+  // Born / adjoint operands of the centre row: the background fields and the
+  // reflectivity (Born) or the gradient accumulator (adjoint, when imaging)
+  Rsrc RA, RB, RCC, RX;
+  unsigned vxl = kDrop, vxs = kDrop;
+  if (T::born || T::adj) {
+    RA = rsrc(a.ua + so, fb); RB = rsrc(a.ub + so, fb); RCC = rsrc(a.uc + so, fb);
+    RX = T::born ? rsrc(m.refl + (long long)shot * m.rstride, fb) : rsrc(a.i1 + so, fb);
+    if (T::born || a.image) { vxl = vcol; vxs = vst; }
   }
 
   float4 q[QL];
@@ -345,6 +362,13 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
     i2b[0] = bld4(RI2, vcol, rowb(zs));
     sub[0] = bld4(RSU, vcs, rowb(zs));
     sqb[0] = bld4(RSQ, vcs, rowb(zs));
+  }
+  float4 xab[2], xbb[2], xcb[2], xxb[2];
+  if (T::born || T::adj) {
+    xab[0] = bld4(RA, vxl, rowb(zs));
+    xbb[0] = bld4(RB, vxl, rowb(zs));
+    xcb[0] = bld4(RCC, vxl, rowb(zs));
+    xxb[0] = bld4(RX, vxl, rowb(zs));
   }
 
   for (int z = zs; z < ze; z += QL) {
@@ -371,6 +395,12 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
         sub[1] = bld4(RSU, vcs, rowb(zz + 1));
         sqb[1] = bld4(RSQ, vcs, rowb(zz + 1));
       }
+      if (T::born || T::adj) {
+        xab[1] = bld4(RA, vxl, rowb(zz + 1));
+        xbb[1] = bld4(RB, vxl, rowb(zz + 1));
+        xcb[1] = bld4(RCC, vxl, rowb(zz + 1));
+        xxb[1] = bld4(RX, vxl, rowb(zz + 1));
+      }
       __builtin_amdgcn_sched_barrier(0);
       // (c) horizontal neighbours through LDS
       const float4 qc = q[(j + NFD) % QL];
@@ -396,10 +426,16 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
           w1 = __builtin_fmaf(cr[jj].c(l), s, w1);
         }
         float o = combine_p(u, compr(u0v, jj), compr(pv, jj), w1, T::f64);
-        if (!T::rcv)  // source (kernel.cu:74-77, 277-280)
+        if (T::src)       // source (kernel.cu:74-77, 277-280)
           o = (zz == sp.x && srcc[jj]) ? o + a.wav : o;
-        else          // data replacement
+        else if (T::rcv)  // data replacement
           o = (zz == a.r.s_z && dval[jj] != 0.0f) ? dval[jj] : o;
+        else if (T::adj)  // residual injection, P * r(k)
+          o = (zz == a.r.s_z && rflag[jj]) ? o + compr(pv, jj) * dval[jj] : o;
+        if (T::born) {    // scattering source m * D2u(k)
+          const float d2u = (compr(xab[0], jj) - 2.0f * compr(xbb[0], jj)) + compr(xcb[0], jj);
+          o = __builtin_fmaf(compr(xxb[0], jj), d2u, o);
+        }
         out[jj] = o;
       }
       const unsigned rowdrop = zz < zhi ? 0u : kDrop;
@@ -423,6 +459,19 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
         bst4(RSU, vss | rowdrop, rz, rsu[0], rsu[1], rsu[2], rsu[3]);
         bst4(RSQ, vss | rowdrop, rz, rsq[0], rsq[1], rsq[2], rsq[3]);
         ukb[0] = ukb[1]; i1b[0] = i1b[1]; i2b[0] = i2b[1]; sub[0] = sub[1]; sqb[0] = sqb[1];
+      }
+      if (T::adj) {  // G += D2u(k+1) * mu(k+2); mu(k+2) is this update's u0
+        float go[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const float d2u = (compr(xab[0], jj) - 2.0f * compr(xbb[0], jj)) + compr(xcb[0], jj);
+          const float gi = compr(xxb[0], jj);
+          go[jj] = colok[jj] ? __builtin_fmaf(d2u, compr(u0v, jj), gi) : gi;
+        }
+        bst4(RX, vxs | rowdrop, rz, go[0], go[1], go[2], go[3]);
+      }
+      if (T::born || T::adj) {
+        xab[0] = xab[1]; xbb[0] = xbb[1]; xcb[0] = xcb[1]; xxb[0] = xxb[1];
       }
 #pragma unroll
       for (int d = 0; d < PD; ++d) {
@@ -507,6 +556,17 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
       if (MODE == FR_FWD) {
         T = combine_p(u, u0, PM[pc], w1, a.f64comb != 0);
         if (gz == sp.x && gx == sp.y) T = T + a.wav;
+      } else if (MODE == FR_BORN) {  // scattered field: m * D2u on the interior, no source
+        T = combine_p(u, u0, PM[pc], w1, a.f64comb != 0);
+        if (gz >= g.n2 && gz < g.nz - g.n2 && gx >= g.n2 && gx < NX - g.n2) {
+          const float d2u = (a.ua[so + pc] - 2.0f * a.ub[so + pc]) + a.uc[so + pc];
+          T = __builtin_fmaf(m.refl[(long long)shot * m.rstride + g.x0 + pc], d2u, T);
+        }
+      } else if (MODE == FR_ADJ) {   // adjoint field: P * r(k) added at the receivers
+        T = combine_p(u, u0, PM[pc], w1, false);
+        int s;
+        if (gz == a.r.s_z && is_receiver(gx, rr, &s))
+          T = T + PM[pc] * a.data[((long long)shot * a.nt + a.k) * a.r.n + s];
       } else {
         int s;
         bool inj = false;
@@ -832,6 +892,40 @@ __global__ void k_stack_add(Grid g, const ShotGeo* geo, int gw, const float* up,
   }
 }
 
+__global__ void k_refl_scatter(Grid g, const ShotGeo* geo, int gw, const float* img, float* dst,
+                               long long rstride) {
+  const int shot = blockIdx.y;
+  const ShotGeo sg = geo[shot];
+  float* D = dst + (long long)shot * rstride;
+  const long long n = (long long)g.nz * g.pitch;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int z = (int)(i / g.pitch), x = (int)(i % g.pitch) - g.x0;
+    float v = 0.0f;
+    const int X = x - g.n2 + sg.x_img;
+    if (z >= g.n2 && z < g.nz - g.n2 && x >= g.n2 && x < sg.nx - g.n2 && X >= 0 && X < gw)
+      v = img[(long long)(z - g.n2) * gw + X];
+    D[i] = v;
+  }
+}
+
+__global__ void k_grad_stack(Grid g, const ShotGeo* geo, int gw, const float* G, const float* pm,
+                             long long mstride, int nshot, float* grad) {
+  const long long n = (long long)g.mnz * gw;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int z = (int)(e / gw), X = (int)(e % gw);
+    float acc = 0.0f;
+    for (int s = 0; s < nshot; ++s) {
+      const int x = X - geo[s].x_img;
+      if (x < 0 || x >= geo[s].mnx) continue;
+      const long long pc = (long long)(z + g.n2) * g.pitch + g.x0 + g.n2 + x;
+      acc = acc + G[(long long)s * g.fstride + pc] / pm[(long long)s * mstride + pc];
+    }
+    grad[e] = acc;
+  }
+}
+
 __global__ void k_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
                                float* img, float* ill) {  // kernel.cu:1042-1059
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nI; e += gridDim.x * blockDim.x) {
@@ -849,7 +943,10 @@ __global__ void k_stack_finish(int nI, const float* s1, const float* s2, int nre
 // launchers
 #define RTM_NFD_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
-int interior_queue_rows(int epi, int nfd) { return 2 * nfd + 1 + (epi == EPI_RCV ? 1 : 2); }
+int interior_queue_rows(int epi, int nfd) {
+  const bool pd1 = epi == EPI_RCV || epi == EPI_BORN || epi == EPI_BORN64 || epi == EPI_ADJ;
+  return 2 * nfd + 1 + (pd1 ? 1 : 2);  // EpiTraits<EPI>::PD
+}
 
 hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s) {
   // rows per wave: the requested segment rounded to a multiple of the queue length
@@ -864,6 +961,9 @@ hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s
     if (epi == EPI_FWD) hipLaunchKernelGGL((k_interior<N, EPI_FWD>), grid, block, 0, s, a);     \
     else if (epi == EPI_FWD64) hipLaunchKernelGGL((k_interior<N, EPI_FWD64>), grid, block, 0, s, a); \
     else if (epi == EPI_SRC) hipLaunchKernelGGL((k_interior<N, EPI_SRC>), grid, block, 0, s, a); \
+    else if (epi == EPI_BORN) hipLaunchKernelGGL((k_interior<N, EPI_BORN>), grid, block, 0, s, a); \
+    else if (epi == EPI_BORN64) hipLaunchKernelGGL((k_interior<N, EPI_BORN64>), grid, block, 0, s, a); \
+    else if (epi == EPI_ADJ) hipLaunchKernelGGL((k_interior<N, EPI_ADJ>), grid, block, 0, s, a); \
     else hipLaunchKernelGGL((k_interior<N, EPI_RCV>), grid, block, 0, s, a);               \
     break;
   switch (nfd) {
@@ -880,6 +980,8 @@ hipError_t launch_frame(int mode, int nfd, FrameArgs a, int nshot, int nrec, hip
 #define CASE(N)                                                                          \
   case N:                                                                                \
     if (mode == FR_FWD) hipLaunchKernelGGL((k_frame<N, FR_FWD>), grid, block, 0, s, a);  \
+    else if (mode == FR_BORN) hipLaunchKernelGGL((k_frame<N, FR_BORN>), grid, block, 0, s, a); \
+    else if (mode == FR_ADJ) hipLaunchKernelGGL((k_frame<N, FR_ADJ>), grid, block, 0, s, a); \
     else hipLaunchKernelGGL((k_frame<N, FR_RCV>), grid, block, 0, s, a);                 \
     break;
   switch (nfd) {
@@ -976,6 +1078,22 @@ hipError_t launch_stack_add(const Grid& g, const ShotGeo* geo, int gw, const flo
                      gw, up, down, s1, s2, nshot);
   return hipGetLastError();
 }
+hipError_t launch_refl_scatter(const Grid& g, const ShotGeo* geo, int gw, const float* img,
+                               float* dst, long long rstride, int nshot, hipStream_t s) {
+  const int ny = rstride == 0 ? 1 : nshot;
+  hipLaunchKernelGGL(k_refl_scatter, dim3(blocks_for((long long)g.nz * g.pitch), ny), dim3(256), 0,
+                     s, g, geo, gw, img, dst, rstride);
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_stack(const Grid& g, const ShotGeo* geo, int gw, const float* G,
+                             const float* pm, long long mstride, int nshot, float* grad,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_stack, dim3(blocks_for((long long)g.mnz * gw)), dim3(256), 0, s, g, geo,
+                     gw, G, pm, mstride, nshot, grad);
+  return hipGetLastError();
+}
+
 hipError_t launch_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
                                float* img, float* illum, hipStream_t s) {
   hipLaunchKernelGGL(k_stack_finish, dim3(blocks_for(nI)), dim3(256), 0, s, nI, s1, s2, nrec, inorm,

@@ -13,11 +13,14 @@ LIB_PATH = os.path.join(_PKG, "lib", "librtm_hip.so")
 
 RTM_STRIP_F32, RTM_STRIP_F16, RTM_STRIP_BF16 = 0, 1, 2
 FIELD_U, FIELD_U_PREV, FIELD_Q, FIELD_I1, FIELD_I2, FIELD_SU, FIELD_SQ = range(7)
-K_FWD_INTERIOR, K_FWD_FRAME, K_SRC_INTERIOR, K_STRIP_RESTORE, K_RCV_INTERIOR, K_RCV_FRAME = range(6)
+(K_FWD_INTERIOR, K_FWD_FRAME, K_SRC_INTERIOR, K_STRIP_RESTORE, K_RCV_INTERIOR, K_RCV_FRAME,
+ K_BORN_INTERIOR, K_BORN_FRAME, K_ADJ_INTERIOR, K_ADJ_FRAME) = range(10)
 KERNEL_NAMES = {
     K_FWD_INTERIOR: "k_interior<EPI_FWD>", K_FWD_FRAME: "k_frame<FR_FWD>",
     K_SRC_INTERIOR: "k_interior<EPI_SRC>", K_STRIP_RESTORE: "k_strip_restore",
     K_RCV_INTERIOR: "k_interior<EPI_RCV>", K_RCV_FRAME: "k_frame<FR_RCV>",
+    K_BORN_INTERIOR: "k_interior<EPI_BORN>", K_BORN_FRAME: "k_frame<FR_BORN>",
+    K_ADJ_INTERIOR: "k_interior<EPI_ADJ>", K_ADJ_FRAME: "k_frame<FR_ADJ>",
 }
 
 
@@ -77,6 +80,12 @@ def lib():
         L.rtm_image_post.argtypes = [vp, C.c_float, C.c_float, vp, vp]
         L.rtm_stack_finish.argtypes = [vp, C.c_int, C.c_int, _fp, vp]
         L.rtm_set_timing.argtypes = [vp, C.c_int]
+        L.rtm_set_reflectivity.argtypes = [vp, _fp]
+        L.rtm_born_begin.argtypes = [vp]
+        L.rtm_born_advance.argtypes = [vp, C.c_int]
+        L.rtm_adjoint_begin.argtypes = [vp]
+        L.rtm_adjoint_advance.argtypes = [vp, C.c_int]
+        L.rtm_adjoint_gradient.argtypes = [vp, _fp]
         L.rtm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
@@ -213,6 +222,47 @@ class Context:
         self._chk(lib().rtm_stack_finish(self.h, int(nrec), int(inorm), img, ill.ctypes.data),
                   "rtm_stack_finish")
         return img.reshape(mnz, self.gw), ill.reshape(mnz, self.gw)
+
+    # ---- least-squares migration operators
+    def set_reflectivity(self, m):
+        """Reflectivity on the stacked image grid [mnz][stack_nx]."""
+        mnz = self.nz - 2 * self.n2
+        m = np.ascontiguousarray(m, np.float32)
+        if m.shape != (mnz, self.gw):
+            raise ValueError(f"reflectivity shape {m.shape}, expected {(mnz, self.gw)}")
+        self._chk(lib().rtm_set_reflectivity(self.h, m.ravel()), "rtm_set_reflectivity")
+
+    def born_begin(self):
+        self._chk(lib().rtm_born_begin(self.h), "rtm_born_begin")
+
+    def born_advance(self, n):
+        self._chk(lib().rtm_born_advance(self.h, int(n)), "rtm_born_advance")
+
+    def born(self):
+        """Born data of the batch [shot][n][nt] for the current reflectivity."""
+        self.born_begin()
+        self.born_advance(self.p.nt)
+        return self.records()
+
+    def adjoint_begin(self):
+        self._chk(lib().rtm_adjoint_begin(self.h), "rtm_adjoint_begin")
+
+    def adjoint_advance(self, n):
+        self._chk(lib().rtm_adjoint_advance(self.h, int(n)), "rtm_adjoint_advance")
+
+    def adjoint_gradient(self):
+        mnz = self.nz - 2 * self.n2
+        out = np.zeros(mnz * self.gw, np.float32)
+        self._chk(lib().rtm_adjoint_gradient(self.h, out), "rtm_adjoint_gradient")
+        return out.reshape(mnz, self.gw)
+
+    def adjoint(self, resid):
+        """Gradient [mnz][stack_nx] of residuals resid[shot][n][nt]; needs a
+        complete forward or Born pass with strips just before."""
+        self.set_gathers(resid)
+        self.adjoint_begin()
+        self.adjoint_advance(self.p.nt)
+        return self.adjoint_gradient()
 
     def autotune(self, nsteps=4):
         """Pick the interior row-segment height by timing forward steps; returns it."""
