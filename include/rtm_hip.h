@@ -203,6 +203,16 @@ int rtm_adjoint_advance(rtm_ctx* ctx, int nsteps);
  * sum over shots, in shot order, of each shot's G/P at its image window. */
 int rtm_adjoint_gradient(rtm_ctx* ctx, float* grad);
 
+/* Data-space vectors of conjugate-gradient iterations stay on the device:
+ * the gathers (residuals read by the adjoint pass) and the records (Born
+ * data of the last Born pass), both [shot][nt][n] over the batch. */
+enum { RTM_DATA_GATHERS = 0, RTM_DATA_RECORDS = 1 };
+/* *out = sum over the batch of x*y in FP64, with a fixed reduction order
+ * (repeatable bit for bit). */
+int rtm_data_dot(rtm_ctx* ctx, int x, int y, double* out);
+/* gathers = gathers + a*records (e.g. the residual update r -= alpha*q). */
+int rtm_data_axpy(rtm_ctx* ctx, float a);
+
 /* Autotuner (not in the reference): times nsteps forward steps of the
  * current batch for each candidate height of the interior kernels' row
  * segments (the kernels round it to a multiple of their register-queue

@@ -77,6 +77,7 @@ struct rtm_ctx {
   size_t refl_cap = 0;
   bool refl_ok = false;
   float* d_grad = nullptr;      // [mnz][gw]
+  double* d_dot = nullptr;      // FP64 dot-product scratch
   bool model_ok = false, wav_ok = false, shots_ok = false, data_ok = false, save_strips = true;
   int zs = 32, nxt = 1;
   // captured migration (hipGraph)
@@ -931,6 +932,35 @@ int rtm_adjoint_gradient(rtm_ctx* c, float* grad) {
                               c->d_grad, c->stream));
   HIPCHK(c, hipMemcpyAsync(grad, c->d_grad, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RTM_OK;
+}
+
+int rtm_data_dot(rtm_ctx* c, int x, int y, double* out) {
+  if (!c || !out) return RTM_E_ARG;
+  const float* v[2] = {c->d_data, c->d_rec};
+  if (x < 0 || x > 1 || y < 0 || y > 1) return fail(c, RTM_E_ARG, "data vector %d, %d", x, y);
+  if ((x == RTM_DATA_RECORDS || y == RTM_DATA_RECORDS) && !c->d_rec)
+    return fail(c, RTM_E_STATE, "context created without records");
+  if ((x == RTM_DATA_GATHERS || y == RTM_DATA_GATHERS) && !c->data_ok)
+    return fail(c, RTM_E_STATE, "rtm_set_gathers first");
+  if (!c->d_dot) {
+    int rc = dalloc_t(c, &c->d_dot, (size_t)dot64_scratch());
+    if (rc != RTM_OK) return rc;
+  }
+  const long long n = (long long)c->nshot * c->p.nt * c->p.n;
+  HIPCHK(c, launch_dot64(v[x], v[y], n, c->d_dot, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, c->d_dot + dot64_scratch() - 1, sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RTM_OK;
+}
+
+int rtm_data_axpy(rtm_ctx* c, float a) {
+  if (!c) return RTM_E_ARG;
+  if (!c->d_rec) return fail(c, RTM_E_STATE, "context created without records");
+  if (!c->data_ok) return fail(c, RTM_E_STATE, "rtm_set_gathers first");
+  const long long n = (long long)c->nshot * c->p.nt * c->p.n;
+  HIPCHK(c, launch_axpy(c->d_data, c->d_rec, a, n, c->stream));
   return RTM_OK;
 }
 

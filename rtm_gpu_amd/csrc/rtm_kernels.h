@@ -160,6 +160,13 @@ hipError_t launch_refl_scatter(const Grid& g, const ShotGeo* geo, int gw, const 
 hipError_t launch_grad_stack(const Grid& g, const ShotGeo* geo, int gw, const float* G,
                              const float* pm, long long mstride, int nshot, float* grad,
                              hipStream_t s);
+// FP64 dot product of float arrays a, b [n]: scratch holds dot64_scratch()
+// doubles; the result lands in scratch[dot64_scratch() - 1].
+int dot64_scratch();
+hipError_t launch_dot64(const float* a, const float* b, long long n, double* scratch,
+                        hipStream_t s);
+// y = y + a*x over n elements
+hipError_t launch_axpy(float* y, const float* x, float a, long long n, hipStream_t s);
 hipError_t launch_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
                                float* img, float* illum, hipStream_t s);
 

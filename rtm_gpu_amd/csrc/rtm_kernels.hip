@@ -926,6 +926,45 @@ __global__ void k_grad_stack(Grid g, const ShotGeo* geo, int gw, const float* G,
   }
 }
 
+// FP64 dot product of two float arrays in two passes with a fixed reduction
+// order (the result does not depend on scheduling): each of the DOT_BLOCKS
+// blocks sums a grid-stride slice in double and reduces it in LDS; one block
+// then adds the partial sums in a fixed tree.
+constexpr int DOT_BLOCKS = 1024;
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  return sh[0];
+}
+__global__ __launch_bounds__(256) void k_dot64_part(const float* a, const float* b, long long n,
+                                                    double* part) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    acc += (double)a[i] * (double)b[i];
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+__global__ __launch_bounds__(256) void k_dot64_final(const double* part, int np, double* out) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[0] = t;
+}
+
+// y = y + a*x (product, then sum)
+__global__ void k_axpy(float* y, const float* x, float a, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = y[i] + a * x[i];
+}
+
 __global__ void k_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
                                float* img, float* ill) {  // kernel.cu:1042-1059
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nI; e += gridDim.x * blockDim.x) {
@@ -1091,6 +1130,21 @@ hipError_t launch_grad_stack(const Grid& g, const ShotGeo* geo, int gw, const fl
                              hipStream_t s) {
   hipLaunchKernelGGL(k_grad_stack, dim3(blocks_for((long long)g.mnz * gw)), dim3(256), 0, s, g, geo,
                      gw, G, pm, mstride, nshot, grad);
+  return hipGetLastError();
+}
+
+int dot64_scratch() { return DOT_BLOCKS + 1; }
+
+hipError_t launch_dot64(const float* a, const float* b, long long n, double* scratch,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_dot64_part, dim3(DOT_BLOCKS), dim3(256), 0, s, a, b, n, scratch);
+  hipLaunchKernelGGL(k_dot64_final, dim3(1), dim3(256), 0, s, scratch, DOT_BLOCKS,
+                     scratch + DOT_BLOCKS);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy(float* y, const float* x, float a, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n)), dim3(256), 0, s, y, x, a, n);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_PKG, "lib", "librtm_hip.so")
 
 RTM_STRIP_F32, RTM_STRIP_F16, RTM_STRIP_BF16 = 0, 1, 2
 FIELD_U, FIELD_U_PREV, FIELD_Q, FIELD_I1, FIELD_I2, FIELD_SU, FIELD_SQ = range(7)
+DATA_GATHERS, DATA_RECORDS = 0, 1
 (K_FWD_INTERIOR, K_FWD_FRAME, K_SRC_INTERIOR, K_STRIP_RESTORE, K_RCV_INTERIOR, K_RCV_FRAME,
  K_BORN_INTERIOR, K_BORN_FRAME, K_ADJ_INTERIOR, K_ADJ_FRAME) = range(10)
 KERNEL_NAMES = {
@@ -86,6 +87,8 @@ def lib():
         L.rtm_adjoint_begin.argtypes = [vp]
         L.rtm_adjoint_advance.argtypes = [vp, C.c_int]
         L.rtm_adjoint_gradient.argtypes = [vp, _fp]
+        L.rtm_data_dot.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.rtm_data_axpy.argtypes = [vp, C.c_float]
         L.rtm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
@@ -263,6 +266,16 @@ class Context:
         self.adjoint_begin()
         self.adjoint_advance(self.p.nt)
         return self.adjoint_gradient()
+
+    def data_dot(self, x, y):
+        """FP64 dot product of device data vectors (DATA_GATHERS / DATA_RECORDS)."""
+        out = C.c_double()
+        self._chk(lib().rtm_data_dot(self.h, int(x), int(y), C.byref(out)), "rtm_data_dot")
+        return out.value
+
+    def data_axpy(self, a):
+        """gathers += a * records on the device."""
+        self._chk(lib().rtm_data_axpy(self.h, float(a)), "rtm_data_axpy")
 
     def autotune(self, nsteps=4):
         """Pick the interior row-segment height by timing forward steps; returns it."""

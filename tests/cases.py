@@ -108,3 +108,56 @@ def rel_l2(a, b):
     if nb == 0:
         return float(np.linalg.norm(a))
     return float(np.linalg.norm(a - b) / nb)
+
+
+class OracleBornOp:
+    """Born operator L and its adjoint on the oracle (one shot), with the
+    interface of rtm_gpu_amd.lsrtm.BornOperator: float32 vectors, float64
+    products.  Test infrastructure."""
+
+    def __init__(self, case, shot=0, prec=32):
+        self.case, self.shot, self.prec = case, shot, prec
+        cfg = case["cfg"]
+        self.n2 = cfg["n2"]
+        self.shape = (cfg["nz"] - 2 * self.n2, cfg["nx"] - 2 * self.n2)
+        self.r = self.q = None
+
+    def _full(self, m):
+        cfg = self.case["cfg"]
+        f = np.zeros((cfg["nz"], cfg["nx"]), np.float32)
+        f[self.n2:-self.n2, self.n2:-self.n2] = m
+        return f
+
+    def born(self, m):
+        e = oracle_engine(self.case, shot=self.shot, prec=self.prec, record=1)
+        e.set_reflectivity(self._full(m))
+        e.born_begin()
+        e.born_advance(self.case["cfg"]["nt"])
+        return e.record()
+
+    def _dot(self, a, b):
+        return float(np.dot(a.astype(np.float64).ravel(), b.astype(np.float64).ravel()))
+
+    def start(self, d):
+        self.r = np.asarray(d, np.float32).reshape(self.case["cfg"]["n"], -1).copy()
+        return self._dot(self.r, self.r)
+
+    def forward(self, p):
+        self.q = self.born(p)
+        return self._dot(self.q, self.q)
+
+    def update_residual(self, alpha):
+        self.r = self.r + (-np.float32(alpha)) * self.q
+        return self._dot(self.r, self.r)
+
+    def adjoint(self):
+        nt = self.case["cfg"]["nt"]
+        e = oracle_engine(self.case, shot=self.shot, prec=self.prec)
+        e.forward_begin()
+        e.forward_advance(nt)
+        e.adjoint_begin(self.r)
+        e.adjoint_advance(nt)
+        return e.field(9)[self.n2:-self.n2, self.n2:-self.n2].astype(np.float32)
+
+    def model_shape(self):
+        return self.shape

@@ -1,0 +1,48 @@
+"""The conjugate-gradient solver of least-squares migration (host logic of
+rtm_gpu_amd.lsrtm), without a GPU: on a dense matrix against the textbook
+least-squares solution, and on the oracle's Born operator (FP32) for the
+monotone decrease of the misfit that CGLS guarantees."""
+import numpy as np
+from scipy.ndimage import gaussian_filter
+
+import cases
+from rtm_gpu_amd import lsrtm
+
+
+def test_cgls_solves_dense_least_squares():
+    rng = np.random.default_rng(4)
+    u, _ = np.linalg.qr(rng.standard_normal((80, 30)))
+    v, _ = np.linalg.qr(rng.standard_normal((30, 30)))
+    A = (u * np.linspace(1.0, 4.0, 30)) @ v.T  # condition number 4
+    d = rng.standard_normal(80).astype(np.float32)
+    m, hist = lsrtm.cgls(lsrtm.MatrixOperator(A), d, 30)
+    exact = np.linalg.lstsq(A.astype(np.float32).astype(np.float64), d.astype(np.float64),
+                            rcond=None)[0]
+    assert np.linalg.norm(m - exact) <= 1e-4 * np.linalg.norm(exact)
+    rr = [h["rr"] for h in hist]
+    assert all(b <= a * (1 + 1e-6) for a, b in zip(rr, rr[1:]))
+    # the minimum misfit is the part of d outside the range of A
+    res = d - A.astype(np.float32) @ exact.astype(np.float32)
+    assert abs(rr[-1] - float(res.astype(np.float64) @ res)) <= 1e-4 * rr[0]
+
+
+def test_cgls_stops_on_zero_data():
+    A = np.eye(5, 3)
+    m, hist = lsrtm.cgls(lsrtm.MatrixOperator(A), np.zeros(5, np.float32), 4)
+    assert not m.any() and len(hist) == 1
+
+
+def test_cgls_on_the_born_operator_decreases_the_misfit():
+    case = cases.build_case(name="cg", ilste=1, nfdmax=4, nt=300, seed=3, mod_nz=44, mod_nx=60)
+    op = cases.OracleBornOp(case)
+    rng = np.random.default_rng(2)
+    mt = np.zeros(op.shape, np.float32)
+    mt[10:-4, 6:-6] = (gaussian_filter(rng.standard_normal((op.shape[0] - 14, op.shape[1] - 12)),
+                                       1.0) * 0.05)
+    d = op.born(mt)
+    m, hist = lsrtm.cgls(op, d, 4)
+    rr = [h["rr"] / hist[0]["rr"] for h in hist]
+    assert all(b < a for a, b in zip(rr, rr[1:])), rr
+    assert rr[-1] < 0.8, rr
+    # the first step is the scaled migration image, which correlates with the model
+    assert float(np.sum(m * mt)) > 0
