@@ -55,11 +55,14 @@ def build_hip(verbose=True):
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "librtm_hip.so")
     srcs = [os.path.join(CSRC, "rtm_kernels.hip"), os.path.join(CSRC, "rtm_capi.hip")]
-    objs = []
-    for s in srcs:
-        o = os.path.join(LIB, os.path.basename(s) + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", s, "-o", o], verbose)
-        objs.append(o)
+    objs = [os.path.join(LIB, os.path.basename(s) + ".o") for s in srcs]
+    # translation units compile in parallel (hipcc is single-threaded)
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", s,
+                                 "-o", o], verbose) for s, o in zip(srcs, objs)]
+        for f in futs:
+            f.result()
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out + ".tmp", *objs], verbose)
     os.replace(out + ".tmp", out)
     for o in objs:

@@ -503,7 +503,7 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
   g.nz = p->nz; g.nx = p->nx; g.n2 = p->n2; g.mnz = mnz; g.mnx = mnx; g.nfd = p->nfdmax;
   const int H = (c->nfd + 3) & ~3;
   int x0 = std::max(4, H);
-  while ((x0 + g.n2) % 4 != 0) ++x0;
+  while ((x0 + g.n2) % 32 != 0) ++x0;  // interior rows start on a 128-byte line
   g.x0 = x0;
   g.pitch = ((x0 + g.nx + 4) + 63) / 64 * 64;
   g.fstride = (long long)g.nz * g.pitch;

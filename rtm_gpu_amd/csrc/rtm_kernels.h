@@ -8,7 +8,8 @@ namespace rtm {
 
 // Padded-grid geometry in device memory.  Cell (z, x) of shot s in a field
 // array lives at s*fstride + z*pitch + x0 + x.  x0 is chosen so that the
-// first interior column n2 is 16-byte aligned, and the row has >= 4 guard
+// first interior column n2 starts a 128-byte cache line (so does every
+// 256-column tile of the interior kernels), and the row has >= 4 guard
 // columns on each side so every float4 row access stays inside its row.
 struct Grid {
   int nz, nx, pitch, x0, n2, mnz, mnx;  // nx, mnx: the widest shot of the batch
