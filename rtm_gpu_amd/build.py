@@ -54,7 +54,7 @@ def _run(cmd, verbose):
 def build_hip(verbose=True):
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "librtm_hip.so")
-    srcs = [os.path.join(CSRC, "rtm_kernels.hip"), os.path.join(CSRC, "rtm_capi.hip")]
+    srcs = [os.path.join(CSRC, f) for f in ("rtm_kernels.hip", "rtm_capi.hip", "rtm_cnn.hip")]
     objs = [os.path.join(LIB, os.path.basename(s) + ".o") for s in srcs]
     # translation units compile in parallel (hipcc is single-threaded)
     from concurrent.futures import ThreadPoolExecutor

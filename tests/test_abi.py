@@ -7,7 +7,7 @@ import re
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = {"rtm_hip.h": "librtm_hip.so", "rtm_host.h": "librtm_host.so"}
+HEADERS = {"rtm_hip.h": "librtm_hip.so", "rtm_host.h": "librtm_host.so", "rtm_cnn.h": "librtm_hip.so"}
 
 
 def _declared(header):
@@ -22,7 +22,7 @@ def _declared(header):
 @pytest.mark.parametrize("header", sorted(HEADERS))
 def test_library_exports_declared_functions(header):
     names = _declared(header)
-    assert len(names) >= 10, names
+    assert len(names) >= 9, names
     lib = C.CDLL(os.path.join(ROOT, "rtm_gpu_amd", "lib", HEADERS[header]))
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, f"{HEADERS[header]} lacks {missing}"
