@@ -207,11 +207,14 @@ int rtm_adjoint_gradient(rtm_ctx* ctx, float* grad);
  * the gathers (residuals read by the adjoint pass) and the records (Born
  * data of the last Born pass), both [shot][nt][n] over the batch. */
 enum { RTM_DATA_GATHERS = 0, RTM_DATA_RECORDS = 1 };
-/* *out = sum over the batch of x*y in FP64, with a fixed reduction order
+/* Over shots [shot0, shot0 + nshot) of the batch (nshot = 0: through the
+ * last shot), so that independent problems batched together keep their own
+ * step lengths: *out = sum of x*y in FP64, with a fixed reduction order
  * (repeatable bit for bit). */
-int rtm_data_dot(rtm_ctx* ctx, int x, int y, double* out);
-/* gathers = gathers + a*records (e.g. the residual update r -= alpha*q). */
-int rtm_data_axpy(rtm_ctx* ctx, float a);
+int rtm_data_dot(rtm_ctx* ctx, int x, int y, int shot0, int nshot, double* out);
+/* gathers = gathers + a*records over shots [shot0, shot0 + nshot) (e.g. the
+ * residual update r -= alpha*q). */
+int rtm_data_axpy(rtm_ctx* ctx, float a, int shot0, int nshot);
 
 /* Autotuner (not in the reference): times nsteps forward steps of the
  * current batch for each candidate height of the interior kernels' row

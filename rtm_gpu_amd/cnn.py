@@ -15,8 +15,8 @@ def _lib():
     L = hip.lib()
     if not _bound:
         vp = C.c_void_p
-        L.rtm_cnn_create.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_longlong, C.c_longlong,
-                                     C.POINTER(vp)]
+        L.rtm_cnn_create.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.c_longlong, C.c_longlong, C.POINTER(vp)]
         L.rtm_cnn_destroy.argtypes = [vp]
         L.rtm_cnn_last_error.argtypes = [vp]
         L.rtm_cnn_last_error.restype = C.c_char_p
@@ -35,14 +35,19 @@ def _lib():
 
 class Deblur:
     """y = x[:, 0] + net(x): nlayer 3x3 convolutions (channels[0] inputs, one
-    output), BF16 on the matrix cores with FP32 accumulation."""
+    output, optional per-layer dilation), BF16 on the matrix cores with FP32
+    accumulation."""
 
-    def __init__(self, channels, max_pixels, max_rows):
+    def __init__(self, channels, max_pixels, max_rows, dilation=None):
         self.channels = list(channels)
+        nl = len(self.channels) - 1
+        self.dilation = list(dilation) if dilation is not None else [1] * nl
+        if len(self.dilation) != nl:
+            raise ValueError(f"{len(self.dilation)} dilations for {nl} layers")
         arr = (C.c_int * len(self.channels))(*self.channels)
+        dil = (C.c_int * nl)(*self.dilation)
         h = C.c_void_p()
-        rc = _lib().rtm_cnn_create(len(self.channels) - 1, arr, int(max_pixels), int(max_rows),
-                                   C.byref(h))
+        rc = _lib().rtm_cnn_create(nl, arr, dil, int(max_pixels), int(max_rows), C.byref(h))
         if rc != 0:
             raise hip.RtmError(f"rtm_cnn_create failed ({rc})")
         self.h = h

@@ -935,7 +935,18 @@ int rtm_adjoint_gradient(rtm_ctx* c, float* grad) {
   return RTM_OK;
 }
 
-int rtm_data_dot(rtm_ctx* c, int x, int y, double* out) {
+// Shots [shot0, shot0 + nshot) of the batch (nshot = 0: through the last).
+static int data_range(rtm_ctx* c, int shot0, int nshot, long long* off, long long* n) {
+  if (shot0 < 0 || shot0 >= c->nshot || nshot < 0 || shot0 + nshot > c->nshot)
+    return fail(c, RTM_E_ARG, "shots %d+%d of %d", shot0, nshot, c->nshot);
+  if (nshot == 0) nshot = c->nshot - shot0;
+  const long long per = (long long)c->p.nt * c->p.n;
+  *off = shot0 * per;
+  *n = nshot * per;
+  return RTM_OK;
+}
+
+int rtm_data_dot(rtm_ctx* c, int x, int y, int shot0, int nshot, double* out) {
   if (!c || !out) return RTM_E_ARG;
   const float* v[2] = {c->d_data, c->d_rec};
   if (x < 0 || x > 1 || y < 0 || y > 1) return fail(c, RTM_E_ARG, "data vector %d, %d", x, y);
@@ -943,24 +954,28 @@ int rtm_data_dot(rtm_ctx* c, int x, int y, double* out) {
     return fail(c, RTM_E_STATE, "context created without records");
   if ((x == RTM_DATA_GATHERS || y == RTM_DATA_GATHERS) && !c->data_ok)
     return fail(c, RTM_E_STATE, "rtm_set_gathers first");
+  long long off = 0, n = 0;
+  int rc = data_range(c, shot0, nshot, &off, &n);
+  if (rc != RTM_OK) return rc;
   if (!c->d_dot) {
-    int rc = dalloc_t(c, &c->d_dot, (size_t)dot64_scratch());
+    rc = dalloc_t(c, &c->d_dot, (size_t)dot64_scratch());
     if (rc != RTM_OK) return rc;
   }
-  const long long n = (long long)c->nshot * c->p.nt * c->p.n;
-  HIPCHK(c, launch_dot64(v[x], v[y], n, c->d_dot, c->stream));
+  HIPCHK(c, launch_dot64(v[x] + off, v[y] + off, n, c->d_dot, c->stream));
   HIPCHK(c, hipMemcpyAsync(out, c->d_dot + dot64_scratch() - 1, sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return RTM_OK;
 }
 
-int rtm_data_axpy(rtm_ctx* c, float a) {
+int rtm_data_axpy(rtm_ctx* c, float a, int shot0, int nshot) {
   if (!c) return RTM_E_ARG;
   if (!c->d_rec) return fail(c, RTM_E_STATE, "context created without records");
   if (!c->data_ok) return fail(c, RTM_E_STATE, "rtm_set_gathers first");
-  const long long n = (long long)c->nshot * c->p.nt * c->p.n;
-  HIPCHK(c, launch_axpy(c->d_data, c->d_rec, a, n, c->stream));
+  long long off = 0, n = 0;
+  int rc = data_range(c, shot0, nshot, &off, &n);
+  if (rc != RTM_OK) return rc;
+  HIPCHK(c, launch_axpy(c->d_data + off, c->d_rec + off, a, n, c->stream));
   return RTM_OK;
 }
 

@@ -1,6 +1,8 @@
 // rtm_cnn.hip -- a small image-domain deblurring network for migrated images
 // (not in the reference): 3x3 convolutions with bias and ReLU, a residual
 // connection from input channel 0, trained with Adam on mean-squared error.
+// Layers may be dilated (tap offsets times the layer's dilation), which
+// widens the receptive field at the same cost.
 // Convolutions run on the matrix cores: operands in BF16, accumulation in
 // FP32 (v_mfma_f32_16x16x16_bf16); master weights, Adam state, loss and
 // gradient reductions stay FP32 / FP64.
@@ -68,6 +70,7 @@ struct ConvArgs {
   const float* resid; // [n][h][w] (mode 1)
   const bf16* mask;   // NHWC, cout_p channels (mode 2)
   int n, h, w, cin_p, kpad, cout_p, mode;
+  int dil;            // dilation: tap (ky, kx) reads (y + (ky-1)*dil, x + (kx-1)*dil)
 };
 
 __global__ __launch_bounds__(256) void k_conv(const ConvArgs a) {
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(256) void k_conv(const ConvArgs a) {
     const int tap = k / a.cin_p, ci = k - tap * a.cin_p;
     s4 av = {0, 0, 0, 0};
     if (tap < 9) {
-      const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+      const int yy = y + (tap / 3 - 1) * a.dil, xx = x + (tap % 3 - 1) * a.dil;
       if (x < a.w && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
         av = load4(a.in + ((nn * a.h + yy) * a.w + xx) * a.cin_p + ci);
     }
@@ -119,7 +122,7 @@ struct WgradArgs {
   const bf16* dz;   // NHWC, dz_p channels
   const bf16* act;  // NHWC, act_p channels (the layer input)
   float* part;      // [rows][WG_STRIDE]
-  int n, h, w, dz_p, act_p;
+  int n, h, w, dz_p, act_p, dil;
 };
 
 __global__ __launch_bounds__(256) void k_wgrad(const WgradArgs a) {
@@ -146,13 +149,13 @@ __global__ __launch_bounds__(256) void k_wgrad(const WgradArgs a) {
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
-      const int yy = y + t / 3 - 1;
+      const int yy = y + (t / 3 - 1) * a.dil;
       // B[k = pixel][n = ci]: lane ci = c16 holds pixels 4*kg + j, shifted by the tap
       s4 bv;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int px = x0 + 4 * kg + j;
-        const int xx = px + t % 3 - 1;
+        const int xx = px + (t % 3 - 1) * a.dil;
         bf16 v = 0;
         if (px < a.w && c16 < a.act_p && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
           v = a.act[((nn * a.h + yy) * a.w + xx) * a.act_p + c16];
@@ -278,6 +281,7 @@ struct rtm_cnn {
   int c[MAXL + 1] = {0};
   int cp[MAXL + 1] = {0};
   int kf[MAXL] = {0}, kb[MAXL] = {0};
+  int dil[MAXL] = {0};
   int off[MAXL + 1] = {0};  // parameter offsets per layer
   int np = 0;
   long long cap_pix = 0, cap_rows = 0;
@@ -365,6 +369,7 @@ int forward_dev(rtm_cnn* c, int n, int h, int w) {
     a.cin_p = c->cp[l];
     a.kpad = c->kf[l];
     a.cout_p = c->cp[l + 1];
+    a.dil = c->dil[l];
     if (l + 1 < c->L) {
       a.mode = 0;
       a.out = c->act[l + 1];
@@ -397,7 +402,7 @@ int loss_grad_dev(rtm_cnn* c, const float* t_dev, int n, int h, int w, double* l
                      c->cp[L], (float)(2.0 / (double)npix), c->dz[L - 1], c->diff);
   CNNCHK(c, hipGetLastError());
   for (int l = L - 1; l >= 0; --l) {
-    WgradArgs g{c->dz[l], c->act[l], c->part, n, h, w, c->cp[l + 1], c->cp[l]};
+    WgradArgs g{c->dz[l], c->act[l], c->part, n, h, w, c->cp[l + 1], c->cp[l], c->dil[l]};
     const long long rows = (long long)n * h;
     hipLaunchKernelGGL(k_wgrad, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, g);
     CNNCHK(c, hipGetLastError());
@@ -416,6 +421,7 @@ int loss_grad_dev(rtm_cnn* c, const float* t_dev, int n, int h, int w, double* l
       a.kpad = c->kb[l];
       a.cout_p = c->cp[l];
       a.mode = 2;
+      a.dil = c->dil[l];
       rc = launch_conv(c, a);
       if (rc != RTM_CNN_OK) return rc;
     }
@@ -448,13 +454,16 @@ int adam_dev(rtm_cnn* c, float lr, float b1, float b2, float eps) {
 
 extern "C" {
 
-int rtm_cnn_create(int nlayer, const int* channels, long long max_pixels, long long max_rows,
-                   rtm_cnn** out) {
+int rtm_cnn_create(int nlayer, const int* channels, const int* dilation, long long max_pixels,
+                   long long max_rows, rtm_cnn** out) {
   if (!channels || !out) return RTM_CNN_E_ARG;
   *out = nullptr;
   if (nlayer < 1 || nlayer > MAXL) return RTM_CNN_E_ARG;
   for (int l = 0; l <= nlayer; ++l)
     if (channels[l] < 1 || channels[l] > MAXC) return RTM_CNN_E_ARG;
+  if (dilation)
+    for (int l = 0; l < nlayer; ++l)
+      if (dilation[l] < 1 || dilation[l] > 64) return RTM_CNN_E_ARG;
   if (channels[nlayer] != 1 || max_pixels < 1 || max_rows < 1) return RTM_CNN_E_ARG;
   rtm_cnn* c = new (std::nothrow) rtm_cnn();
   if (!c) return RTM_CNN_E_NOMEM;
@@ -464,6 +473,7 @@ int rtm_cnn_create(int nlayer, const int* channels, long long max_pixels, long l
     c->cp[l] = pad4(channels[l]);
   }
   for (int l = 0; l < nlayer; ++l) {
+    c->dil[l] = dilation ? dilation[l] : 1;
     c->kf[l] = pad16(9 * c->cp[l]);
     c->kb[l] = pad16(9 * c->cp[l + 1]);
     c->off[l + 1] = c->off[l] + c->c[l + 1] * c->c[l] * 9 + c->c[l + 1];

@@ -1,0 +1,251 @@
+"""Learned image-domain deblurring of migrated images, against least-squares
+migration.  Not in the reference, which stops at the migrated image
+(kernel.cu:935-1059).
+
+The migrated image of Born data is H m = L^T L m: the reflectivity m blurred
+by the Hessian of Born modelling.  Least-squares migration (lsrtm.cgls)
+undoes the blur iteratively, at two wave-equation passes per iteration.  A
+small convolutional network (cnn.Deblur) learns an approximate inverse of H
+in the image domain instead:
+
+  training pairs   random reflectivities m_i (the high-pass velocity
+                   contrast (v^2 - v0^2)/v0^2 of faulted layered models drawn
+                   with other seeds than the evaluated one) -> Born data
+                   d_i = L m_i -> input x_i = L^T d_i = H m_i, target the
+                   image of cg_iters CGLS iterations on d_i (or m_i itself
+                   with target="true"), all by this engine, many pairs per
+                   launch (each panel of a wide reflectivity is one group of
+                   shots at its own image columns, with its own CG step
+                   lengths);
+  input channels   c * x (c: the least-squares scalar between images and
+                   reflectivities of the training set), the normalised log
+                   source illumination, and depth;
+  output           m ~ c x + net(x) (zero net: the scaled migrated image,
+                   the plain-migration fallback).
+
+evaluate() compares the network's estimate with 10 CGLS iterations of the
+same engine on the evaluated model: relative L2 between the two images, the
+Born re-modelled data residual ||L m - d|| / ||d|| of each, and wall times."""
+import time
+
+import numpy as np
+
+from . import cnn, hip, host, lsrtm, synth
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def build_survey(mod_nz=200, mod_nx=400, h=10.0, tao=0.001, f0=15.0, nt=2000, nshot=12,
+                 nfdmax=6, n2=30, seed=2, src_depth=10.0):
+    """Surface survey on the seeded faulted model (column-major [x][z]): nshot
+    sources 10 m deep, evenly spaced, a receiver at every surface point, the
+    smoothed model as migration background."""
+    cfg = host.make_config(nfdmax=nfdmax, nfdmin=2, n2=n2, f0=f0, fmax=2 * f0 + 1, df=1.0,
+                           nthita=1000, eps=1e-5, dv=1.0, ilste=0, ifv=0, whitecoe=1e-4, hz=h,
+                           tao=tao, inorm=1, icompen=1, h=h, tao1=tao, mod_nz=mod_nz,
+                           mod_nx=mod_nx, nt1=nt, s_l1=1, s_z1=1, n=mod_nx, ds=1,
+                           r_x1=mod_nx // 2, nrec=nshot, dr=1)
+    d = host.derive(cfg)
+    true_raw = synth.faulted_rvsp(seed=seed, mod_nz=mod_nz, mod_nx=mod_nx, h=h)
+    mig_raw = synth.smooth(true_raw, 4.0)
+    v, r1 = host.pad_model(mig_raw, mod_nz, mod_nx, n2, 0, tao, h)
+    vmin, vmax, nvel = host.vrange(v, 1.0)
+    need = host.bins_needed(v, vmin, 1.0, nvel)
+    _, index, c = host.design_ls(1000, nfdmax, 2, nvel, tao, h, 1.0, 1e-5, cfg.fmax, vmin, 1.0,
+                                 need, d.hzx)
+    cols = np.linspace(0, mod_nx - 1, nshot + 2)[1:-1].round().astype(int)
+    r_u = [host.source_row(src_depth, h, n2)] * nshot
+    r_x = [int(d.s_l + cx) for cx in cols]
+    return dict(cfg=cfg, d=d, v=v, r1=r1, c=c, index=index, nvel=nvel, vmin=vmin, vmax=vmax,
+                wav=host.wavelet_table(d.nt, tao, f0), w=host.blend_weights(n2), r_u=r_u,
+                r_x=r_x, nshot=nshot, mod_nz=mod_nz, mod_nx=mod_nx, n2=n2, seed=seed,
+                m_true=reflectivity(true_raw, mig_raw))
+
+
+def reflectivity(raw, raw_smooth):
+    """Born scattering strength (v^2 - v0^2) / v0^2 on the image grid [z][x]."""
+    v = raw.astype(np.float64).T
+    v0 = raw_smooth.astype(np.float64).T
+    return ((v * v - v0 * v0) / (v0 * v0)).astype(np.float32)
+
+
+def random_reflectivity(S, seed):
+    raw = synth.faulted_rvsp(seed=seed, mod_nz=S["mod_nz"], mod_nx=S["mod_nx"],
+                             h=float(S["cfg"].h))
+    return reflectivity(raw, synth.smooth(raw, 4.0))
+
+
+def context(S, panels=1):
+    """Context of `panels` copies of the survey's shots; copy p images columns
+    [p*mnx, (p+1)*mnx) of a reflectivity panels * mnx wide."""
+    d, cfg = S["d"], S["cfg"]
+    ns = S["nshot"] * panels
+    ctx = hip.Context(nz=d.nz, nx=d.nx, n2=cfg.n2, nfdmax=cfg.nfdmax, nt=d.nt, nt2=d.nt2,
+                      ilste=0, icompen=1, s_l=d.s_l, s_r=d.s_r, s_z=d.s_z, ds=1, n=cfg.n,
+                      tao2=d.tao2, h2=d.h2, taoh=d.taoh, taoh2=d.taoh2, hzx2_1=d.hzx2_1,
+                      max_shots=ns, record=1, stack_nx=S["mod_nx"] * panels)
+    ctx.set_model(S["v"], S["r1"], S["w"], S["c"], S["index"], S["nvel"], S["vmin"], 1.0)
+    ctx.set_wavelet(S["wav"])
+    if panels == 1:
+        ctx.set_shots(S["r_u"], S["r_x"])
+    else:
+        ctx.set_shot_geometry([dict(nx=d.nx, r_u=S["r_u"][s], r_x=S["r_x"][s], s_l=d.s_l,
+                                    s_r=d.s_r, n=cfg.n, x_img=p * S["mod_nx"])
+                               for p in range(panels) for s in range(S["nshot"])])
+    return ctx
+
+
+def normal_images(ctx, m_panels):
+    """H m for each panel (shots of one panel image their own columns)."""
+    npan, mnz, mnx = m_panels.shape
+    ctx.set_reflectivity(np.ascontiguousarray(m_panels.transpose(1, 0, 2).reshape(mnz, npan * mnx)))
+    ctx.born_begin()
+    ctx.born_advance(ctx.p.nt)
+    ctx.gathers_from_records()
+    ctx.adjoint_begin()
+    ctx.adjoint_advance(ctx.p.nt)
+    g = ctx.adjoint_gradient()
+    return np.ascontiguousarray(g.reshape(mnz, npan, mnx).transpose(1, 0, 2))
+
+
+def illumination(S):
+    """Source illumination sum_k u(k)^2 of the survey's shots, stacked [z][x]."""
+    ctx = context(S)
+    ctx.set_gathers(np.zeros((S["nshot"], S["cfg"].n, S["d"].nt), np.float32))
+    ctx.migrate()
+    _, down = ctx.image_post(float(S["vmax"]), 0.0)
+    return down.sum(axis=0)
+
+
+def features(S, img, c, illum):
+    """Network input [n][3][z][x]: c * image, log illumination in [0, 1], depth."""
+    img = np.asarray(img, np.float32)
+    if img.ndim == 2:
+        img = img[None]
+    n, mnz, mnx = img.shape
+    li = np.log10(np.maximum(illum / illum.max(), 1e-6)) / 6.0 + 1.0
+    depth = np.broadcast_to(np.linspace(0.0, 1.0, mnz, dtype=np.float32)[:, None], (mnz, mnx))
+    x = np.empty((n, 3, mnz, mnx), np.float32)
+    x[:, 0] = np.float32(c) * img
+    x[:, 1] = li
+    x[:, 2] = depth
+    return x
+
+
+def make_pairs(S, npairs, panels, seed0=100, target="cg", cg_iters=10):
+    """Inputs H m_i and targets (CGLS images of L m_i, or m_i) [npairs][mnz][mnx]."""
+    ctx = context(S, panels)
+    ms, xs = [], []
+    for b in range(0, npairs, panels):
+        k = min(panels, npairs - b)
+        m = np.stack([random_reflectivity(S, seed0 + b + i) for i in range(panels)])
+        if k < panels:
+            m[k:] = 0.0
+        x = normal_images(ctx, m)
+        if target == "cg":
+            # the gathers still hold the panels' Born data (normal_images)
+            m, _ = lsrtm.cgls(lsrtm.BornOperator(ctx, groups=panels), None, cg_iters)
+        ms.append(m[:k])
+        xs.append(x[:k])
+    return np.concatenate(xs), np.concatenate(ms)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def run(S=None, npairs=128, panels=32, epochs=600, batch=8, lr=1e-3,
+        channels=(3,) + (16,) * 11 + (1,), dilation=(1, 2, 4, 8, 16, 32, 16, 8, 4, 2, 1, 1),
+        cg_iters=10, seed=0, target="cg"):
+    """Train on engine-made pairs and compare with CGLS; returns a results dict."""
+    S = S or build_survey()
+    res = dict(grid=f"{S['mod_nz']}x{S['mod_nx']}", nt=int(S["d"].nt), shots=S["nshot"],
+               npairs=npairs, epochs=epochs, channels=list(channels),
+               dilation=list(dilation) if dilation else None, target=target)
+    t = time.time()
+    X, M = make_pairs(S, npairs, panels, target=target, cg_iters=cg_iters)
+    res["pairs_s"] = time.time() - t
+    c = float(np.sum(X.astype(np.float64) * M) / np.sum(X.astype(np.float64) ** 2))
+    res["scale"] = c
+    illum = illumination(S)
+    XF = features(S, X, c, illum)
+    mnz, mnx = M.shape[1:]
+    net = cnn.Deblur(list(channels), batch * mnz * mnx, batch * mnz, dilation=dilation)
+    net.init_params(seed=seed)
+    t = time.time()
+    hist = net.fit(XF, M, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    res["train_s"] = time.time() - t
+    res["loss_first"], res["loss_last"] = float(hist[0]), float(hist[-1])
+    res["loss_identity"] = float(np.mean((XF[:, 0] - M) ** 2))
+    log(f"[deblur] pairs {res['pairs_s']:.1f}s, training {res['train_s']:.1f}s, loss "
+        f"{res['loss_identity']:.3e} (scaled image) -> {res['loss_last']:.3e}")
+
+    # the evaluated model: Born data of its reflectivity, then three estimates
+    ctx = context(S)
+    op = lsrtm.BornOperator(ctx)
+    m_true = S["m_true"]
+    ctx.set_reflectivity(m_true)
+    dobs = ctx.born()
+    hip_sync = ctx.sync
+    t = time.time()
+    ctx.forward_begin(True)
+    ctx.forward_advance(ctx.p.nt)
+    img = ctx.adjoint(dobs)
+    hip_sync()
+    t_img = time.time() - t
+    t = time.time()
+    m_net = net.forward(features(S, img, c, illum))[0]
+    t_net = time.time() - t
+    m_rtm = np.float32(c) * img
+    t = time.time()
+    m_cg, ch = lsrtm.cgls(op, dobs, cg_iters)
+    t_cg = time.time() - t
+
+    def resid(m):
+        ctx.set_reflectivity(m)
+        r = ctx.born() - dobs
+        return float(np.linalg.norm(r.astype(np.float64)) / np.linalg.norm(dobs.astype(np.float64)))
+
+    res.update(
+        t_image_s=t_img, t_net_s=t_net, t_cg_s=t_cg,
+        speedup_vs_cg=t_cg / (t_img + t_net),
+        rel_net_vs_cg=rel(m_net, m_cg), rel_rtm_vs_cg=rel(m_rtm, m_cg),
+        rel_net_vs_true=rel(m_net, m_true), rel_rtm_vs_true=rel(m_rtm, m_true),
+        rel_cg_vs_true=rel(m_cg, m_true),
+        resid_rtm=resid(m_rtm), resid_net=resid(m_net), resid_cg=resid(m_cg),
+        cg_rr=[h["rr"] / ch[0]["rr"] for h in ch])
+    log(f"[deblur] image {t_img:.2f}s + net {t_net:.3f}s vs CGLS({cg_iters}) {t_cg:.2f}s; "
+        f"rel L2 to CGLS: net {res['rel_net_vs_cg']:.3f}, scaled RTM {res['rel_rtm_vs_cg']:.3f}; "
+        f"data residual: RTM {res['resid_rtm']:.3f}, net {res['resid_net']:.3f}, "
+        f"CGLS {res['resid_cg']:.3f}")
+    return res, dict(m_true=m_true, m_rtm=m_rtm, m_net=m_net, m_cg=m_cg)
+
+
+def main():
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="deblurring network vs CGLS on the faulted model")
+    ap.add_argument("--npairs", type=int, default=128)
+    ap.add_argument("--panels", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--cg-iters", type=int, default=10)
+    ap.add_argument("--shots", type=int, default=12)
+    ap.add_argument("--nt", type=int, default=2000)
+    ap.add_argument("--target", default="cg", choices=["cg", "true"],
+                    help="train toward the CGLS image of each pair or its reflectivity")
+    ap.add_argument("--save", default="", help="write the four images to this .npz")
+    a = ap.parse_args()
+    S = build_survey(nshot=a.shots, nt=a.nt)
+    res, im = run(S, npairs=a.npairs, panels=a.panels, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                  cg_iters=a.cg_iters, target=a.target)
+    if a.save:
+        np.savez_compressed(a.save, **im)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -87,8 +87,8 @@ def lib():
         L.rtm_adjoint_begin.argtypes = [vp]
         L.rtm_adjoint_advance.argtypes = [vp, C.c_int]
         L.rtm_adjoint_gradient.argtypes = [vp, _fp]
-        L.rtm_data_dot.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
-        L.rtm_data_axpy.argtypes = [vp, C.c_float]
+        L.rtm_data_dot.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.rtm_data_axpy.argtypes = [vp, C.c_float, C.c_int, C.c_int]
         L.rtm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
@@ -267,15 +267,17 @@ class Context:
         self.adjoint_advance(self.p.nt)
         return self.adjoint_gradient()
 
-    def data_dot(self, x, y):
-        """FP64 dot product of device data vectors (DATA_GATHERS / DATA_RECORDS)."""
+    def data_dot(self, x, y, shot0=0, nshot=0):
+        """FP64 dot product of device data vectors (DATA_GATHERS / DATA_RECORDS)
+        over shots [shot0, shot0 + nshot) (nshot 0: through the last)."""
         out = C.c_double()
-        self._chk(lib().rtm_data_dot(self.h, int(x), int(y), C.byref(out)), "rtm_data_dot")
+        self._chk(lib().rtm_data_dot(self.h, int(x), int(y), int(shot0), int(nshot), C.byref(out)),
+                  "rtm_data_dot")
         return out.value
 
-    def data_axpy(self, a):
-        """gathers += a * records on the device."""
-        self._chk(lib().rtm_data_axpy(self.h, float(a)), "rtm_data_axpy")
+    def data_axpy(self, a, shot0=0, nshot=0):
+        """gathers += a * records on the device, over shots [shot0, shot0 + nshot)."""
+        self._chk(lib().rtm_data_axpy(self.h, float(a), int(shot0), int(nshot)), "rtm_data_axpy")
 
     def autotune(self, nsteps=4):
         """Pick the interior row-segment height by timing forward steps; returns it."""

@@ -32,32 +32,58 @@ def dot64(a, b):
 
 class BornOperator:
     """L and L^T of one shot batch of a hip.Context (created with record = 1),
-    with the data-space vectors resident in the context."""
+    with the data-space vectors resident in the context.
 
-    def __init__(self, ctx):
+    groups > 1 batches independent problems: the batch's shots split into
+    `groups` equal runs, run g imaging columns [g*w, (g+1)*w) of the stacked
+    image (w = stack width / groups), and every inner product and update is
+    taken per group, so each problem keeps its own step lengths.  Model
+    vectors are then [groups][mnz][w]."""
+
+    def __init__(self, ctx, groups=1):
         if not ctx.p.record:
             raise hip.RtmError("the Born operator needs a context created with record=1")
+        if groups < 1 or ctx.nshot % groups or ctx.gw % groups:
+            raise hip.RtmError(f"{ctx.nshot} shots / {ctx.gw} columns do not split into {groups}")
         self.ctx = ctx
+        self.groups = groups
+        self.per = ctx.nshot // groups
         self._background = False  # a forward / Born pass has left its end state and strips
 
+    def _dots(self, x, y):
+        if self.groups == 1:
+            return self.ctx.data_dot(x, y)
+        return np.array([self.ctx.data_dot(x, y, g * self.per, self.per)
+                         for g in range(self.groups)])
+
     def start(self, d):
-        """r = d (observed data [shot][n][nt]); returns <d, d>."""
-        self.ctx.set_gathers(d)
-        return self.ctx.data_dot(hip.DATA_GATHERS, hip.DATA_GATHERS)
+        """r = d (observed data [shot][n][nt]; None: the context's gathers
+        already hold it); returns <d, d>."""
+        if d is not None:
+            self.ctx.set_gathers(d)
+        return self._dots(hip.DATA_GATHERS, hip.DATA_GATHERS)
 
     def forward(self, p):
         """q = L p into the records; returns <q, q>."""
         ctx = self.ctx
+        p = np.asarray(p, np.float32)
+        if self.groups > 1:
+            G, mnz, w = p.shape
+            p = p.transpose(1, 0, 2).reshape(mnz, G * w)
         ctx.set_reflectivity(p)
         ctx.born_begin()
         ctx.born_advance(ctx.p.nt)
         self._background = True
-        return ctx.data_dot(hip.DATA_RECORDS, hip.DATA_RECORDS)
+        return self._dots(hip.DATA_RECORDS, hip.DATA_RECORDS)
 
     def update_residual(self, alpha):
         """r -= alpha q; returns <r, r>."""
-        self.ctx.data_axpy(-np.float32(alpha))
-        return self.ctx.data_dot(hip.DATA_GATHERS, hip.DATA_GATHERS)
+        if self.groups == 1:
+            self.ctx.data_axpy(-np.float32(alpha))
+        else:
+            for g in range(self.groups):
+                self.ctx.data_axpy(-np.float32(alpha[g]), g * self.per, self.per)
+        return self._dots(hip.DATA_GATHERS, hip.DATA_GATHERS)
 
     def adjoint(self):
         """s = L^T r (the stacked gradient)."""
@@ -68,10 +94,17 @@ class BornOperator:
         ctx.adjoint_begin()
         ctx.adjoint_advance(ctx.p.nt)
         self._background = False
-        return ctx.adjoint_gradient()
+        g = ctx.adjoint_gradient()
+        if self.groups > 1:
+            mnz = g.shape[0]
+            g = np.ascontiguousarray(g.reshape(mnz, self.groups, -1).transpose(1, 0, 2))
+        return g
 
     def model_shape(self):
-        return (self.ctx.nz - 2 * self.ctx.n2, self.ctx.gw)
+        mnz = self.ctx.nz - 2 * self.ctx.n2
+        if self.groups == 1:
+            return (mnz, self.ctx.gw)
+        return (self.groups, mnz, self.ctx.gw // self.groups)
 
 
 class MatrixOperator:
@@ -105,28 +138,46 @@ class MatrixOperator:
 def cgls(op, d, niter, callback=None):
     """niter CGLS iterations from m = 0.  Returns (m, history) with history a
     list of dicts {iter, rr (= ||r||^2), ss (= ||L^T r||^2), alpha, beta}.
-    Stops early when L^T r or L p vanishes."""
-    rr = op.start(d)
+    An operator with groups > 1 runs one CGLS per group (scalars become
+    per-group arrays); a group whose L^T r or L p vanishes stops moving."""
+    G = getattr(op, "groups", 1)
+
+    def gdot(a, b):
+        a2 = np.asarray(a).reshape(G, -1)
+        b2 = np.asarray(b).reshape(G, -1)
+        return np.array([dot64(a2[g], b2[g]) for g in range(G)])
+
+    def scale(v, coef):  # float32 per-group multiple of a model vector
+        c = np.asarray(coef, np.float64).astype(np.float32).reshape((G,) + (1,) * (v.ndim - 1))
+        return c * v if G > 1 else np.float32(coef[0]) * v
+
+    def scalar(v):
+        return v if G > 1 else float(v[0])
+
+    rr = np.atleast_1d(op.start(d))
     s = op.adjoint()
     p = s.copy()
-    gamma = dot64(s, s)
+    gamma = gdot(s, s)
     m = np.zeros(op.model_shape(), np.float32)
-    hist = [dict(iter=0, rr=rr, ss=gamma, alpha=0.0, beta=0.0)]
+    hist = [dict(iter=0, rr=scalar(rr), ss=scalar(gamma), alpha=scalar(np.zeros(G)),
+                 beta=scalar(np.zeros(G)))]
     for it in range(1, niter + 1):
-        if gamma == 0.0:
+        if not (gamma > 0.0).any():
             break
-        qq = op.forward(p)
-        if qq == 0.0:
+        qq = np.atleast_1d(op.forward(p))
+        ok = (gamma > 0.0) & (qq > 0.0)
+        if not ok.any():
             break
-        alpha = gamma / qq
-        m = m + np.float32(alpha) * p
-        rr = op.update_residual(alpha)
+        alpha = np.where(ok, gamma / np.where(ok, qq, 1.0), 0.0)
+        m = m + scale(p, alpha)
+        rr = np.atleast_1d(op.update_residual(scalar(alpha)))
         s = op.adjoint()
-        gamma_new = dot64(s, s)
-        beta = gamma_new / gamma
-        p = s + np.float32(beta) * p
+        gamma_new = gdot(s, s)
+        beta = np.where(gamma > 0.0, gamma_new / np.where(gamma > 0.0, gamma, 1.0), 0.0)
+        p = s + scale(p, beta)
         gamma = gamma_new
-        hist.append(dict(iter=it, rr=rr, ss=gamma, alpha=alpha, beta=beta))
+        hist.append(dict(iter=it, rr=scalar(rr), ss=scalar(gamma), alpha=scalar(alpha),
+                         beta=scalar(beta)))
         if callback is not None:
             callback(it, m, hist[-1])
     return m, hist

@@ -33,13 +33,14 @@ def unpack(p, ch):
     return out
 
 
-def reference(p, ch, x, t=None):
+def reference(p, ch, x, t=None, dil=None):
     """Forward (and with targets, MSE loss and gradient) with BF16 rounding."""
     layers = unpack(p, ch)
+    dil = dil or [1] * len(layers)
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a))  # noqa: E731
     acts = [bf(x)]
     for i, (W, b) in enumerate(layers):
-        z = F.conv2d(T(acts[-1]), T(bf(W)), T(b), padding=1).numpy()
+        z = F.conv2d(T(acts[-1]), T(bf(W)), T(b), padding=dil[i], dilation=dil[i]).numpy()
         if i + 1 < len(layers):
             acts.append(bf(np.maximum(z, 0.0)))
         else:
@@ -52,17 +53,19 @@ def reference(p, ch, x, t=None):
     dz = bf((2.0 / n) * (y - t))[:, None]
     for i in range(len(layers) - 1, -1, -1):
         W, b = layers[i]
-        gw = torch.nn.grad.conv2d_weight(T(acts[i]), W.shape, T(dz), padding=1).numpy()
+        gw = torch.nn.grad.conv2d_weight(T(acts[i]), W.shape, T(dz), padding=dil[i],
+                                         dilation=dil[i]).numpy()
         grads[i] = np.concatenate([gw.ravel(), dz.sum(axis=(0, 2, 3))])
         if i > 0:
-            dx = torch.nn.grad.conv2d_input(acts[i].shape, T(bf(W)), T(dz), padding=1).numpy()
+            dx = torch.nn.grad.conv2d_input(acts[i].shape, T(bf(W)), T(dz), padding=dil[i],
+                                            dilation=dil[i]).numpy()
             dz = bf(dx * (acts[i] > 0))
     return y, loss, np.concatenate(grads)
 
 
-def _net(ch, seed=0, npix=4096 * 4, rows=256):
+def _net(ch, seed=0, npix=4096 * 4, rows=256, dil=None):
     from rtm_gpu_amd import cnn
-    net = cnn.Deblur(ch, npix, rows)
+    net = cnn.Deblur(ch, npix, rows, dilation=dil)
     p = net.init_params(seed=seed, last_zero=False)
     p[:] += np.random.default_rng(seed + 1).standard_normal(p.size).astype(np.float32) * 0.01
     net.set_params(p)
@@ -109,6 +112,21 @@ def test_deep_forward_and_gradient(ch):
         k = ch[i + 1] * ch[i] * 9 + ch[i + 1]
         assert _rel(g[o:o + k], gr[o:o + k]) < 1e-2, i
         o += k
+
+
+@pytest.mark.parametrize("dil", [[1, 2, 4, 1], [3, 1, 8, 2]])
+def test_dilated_forward_and_gradient(dil):
+    ch = [3, 16, 8, 16, 1]
+    net, p = _net(ch, seed=6, dil=dil)
+    x, t = _data(2, ch[0], 30, 45, seed=5)
+    y = net.forward(x)
+    yr, lr, gr = reference(p, ch, x, t, dil)
+    print(f"dilation {dil}: forward rel {_rel(y, yr):.2e}", flush=True)
+    assert _rel(y, yr) < 1e-4
+    loss, g = net.loss_grad(x, t)
+    print(f"dilation {dil}: gradient rel {_rel(g, gr):.2e}")
+    assert abs(loss - lr) <= 1e-4 * lr
+    assert _rel(g, gr) < 5e-3
 
 
 def test_adam_step():

@@ -46,3 +46,40 @@ def test_cgls_on_the_born_operator_decreases_the_misfit():
     assert rr[-1] < 0.8, rr
     # the first step is the scaled migration image, which correlates with the model
     assert float(np.sum(m * mt)) > 0
+
+
+class _Blocks:
+    """G independent dense problems with the grouped operator interface."""
+
+    def __init__(self, As):
+        self.ops = [lsrtm.MatrixOperator(A) for A in As]
+        self.groups = len(As)
+
+    def start(self, d):
+        return np.array([o.start(di) for o, di in zip(self.ops, d)])
+
+    def forward(self, p):
+        return np.array([o.forward(pi) for o, pi in zip(self.ops, p)])
+
+    def update_residual(self, alpha):
+        return np.array([o.update_residual(a) for o, a in zip(self.ops, alpha)])
+
+    def adjoint(self):
+        return np.stack([o.adjoint() for o in self.ops])
+
+    def model_shape(self):
+        return (self.groups, self.ops[0].A.shape[1])
+
+
+def test_grouped_cgls_equals_independent_runs():
+    rng = np.random.default_rng(7)
+    As = [rng.standard_normal((40, 12)) * (1 + g) for g in range(3)]
+    ds = [rng.standard_normal(40).astype(np.float32) for _ in range(3)]
+    ds[2][:] = 0.0  # a group with nothing to fit stays at zero
+    m, hist = lsrtm.cgls(_Blocks(As), np.stack(ds), 6)
+    for g in range(3):
+        mg, hg = lsrtm.cgls(lsrtm.MatrixOperator(As[g]), ds[g], 6)
+        assert np.array_equal(m[g], mg), g
+        for a, b in zip(hist, hg):
+            assert a["rr"][g] == b["rr"]
+    assert not m[2].any()
