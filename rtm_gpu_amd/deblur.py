@@ -146,9 +146,12 @@ def make_pairs(S, npairs, panels, seed0=100, target="cg", cg_iters=10):
         x = normal_images(ctx, m)
         if target == "cg":
             # the gathers still hold the panels' Born data (normal_images)
-            m, _ = lsrtm.cgls(lsrtm.BornOperator(ctx, groups=panels), None, cg_iters)
+            m, _ = lsrtm.cgls(lsrtm.BornOperator(ctx, groups=panels), None, cg_iters,
+                              callback=lambda it, *_: log(f"[deblur] pairs {b}..{b + k - 1}: "
+                                                          f"CG iteration {it}"))
         ms.append(m[:k])
         xs.append(x[:k])
+        log(f"[deblur] pairs {b + k} of {npairs} done")
     return np.concatenate(xs), np.concatenate(ms)
 
 
@@ -175,7 +178,11 @@ def run(S=None, npairs=128, panels=32, epochs=600, batch=8, lr=1e-3,
     net = cnn.Deblur(list(channels), batch * mnz * mnx, batch * mnz, dilation=dilation)
     net.init_params(seed=seed)
     t = time.time()
-    hist = net.fit(XF, M, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    hist = []
+    for e0 in range(0, epochs, 50):  # chunks, so progress shows; Adam state carries over
+        hist.extend(net.fit(XF, M, epochs=min(50, epochs - e0), batch=batch, lr=lr,
+                            seed=seed + e0))
+        log(f"[deblur] epoch {e0 + min(50, epochs - e0)} of {epochs}: loss {hist[-1]:.3e}")
     res["train_s"] = time.time() - t
     res["loss_first"], res["loss_last"] = float(hist[0]), float(hist[-1])
     res["loss_identity"] = float(np.mean((XF[:, 0] - M) ** 2))
