@@ -146,8 +146,8 @@ def run_apertures(a):
         for j, i in enumerate(idx):
             v, r1 = A["window"](A["mig"], i)
             ctx.set_shot_model(j, v, r1)
-        if zseg <= 0:  # the tuner's forward steps leave the uploaded gathers alone
-            zseg = ctx.autotune(4)
+        if a.zseg <= 0 and b0 == 0:  # the tuner leaves the uploaded gathers alone
+            zseg = ctx.autotune(8)
             log(f"[apertures] autotune: {zseg} rows per wave")
         ctx.sync()
         t0 = time.perf_counter()
@@ -170,7 +170,7 @@ def run_apertures(a):
     for i in sample:
         gm = A["geoms"][i]
         one = hip.Context(**common, nx=gm["nx"], s_l=gm["s_l"], s_r=gm["s_r"], n=gm["n"],
-                          max_shots=1, record=1, zseg=zseg)
+                          max_shots=1, record=1, zseg=a.zseg)
         v, r1 = A["window"](A["true"], i)
         one.set_model(v, r1, A["w"], A["c"], A["index"], A["nvel"], A["vmin"], 1.0)
         one.set_wavelet(A["wav"])
@@ -180,6 +180,8 @@ def run_apertures(a):
         one.gathers_from_records()
         v, r1 = A["window"](A["mig"], i)
         one.set_model(v, r1, A["w"], A["c"], A["index"], A["nvel"], A["vmin"], 1.0)
+        if a.zseg <= 0:  # each one-shot context tuned for its own width
+            one.autotune(8)
         one.sync()
         t0 = time.perf_counter()
         one.migrate()
@@ -288,6 +290,50 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
 
 
+def copy_bandwidth(gib=2, reps=5):
+    """Device-to-device copy rate (read + write bytes / time) of a gib-GiB
+    buffer, timed with HIP events: the bandwidth this GPU delivers to a plain
+    streaming kernel at the moment, next to which the stencil's fraction of
+    the 8 TB/s peak reads.  Uses the HIP runtime that librtm_hip.so loaded
+    (a second runtime in the process would not see the GPU)."""
+    import ctypes as C
+    from rtm_gpu_amd import hip
+    hip.lib()
+    path = next(l.split()[-1] for l in open("/proc/self/maps") if "libamdhip64" in l)
+    rt = C.CDLL(path)
+    vp = C.c_void_p
+    rt.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
+    rt.hipFree.argtypes = [vp]
+    rt.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+    rt.hipEventCreate.argtypes = [C.POINTER(vp)]
+    rt.hipEventRecord.argtypes = [vp, vp]
+    rt.hipEventSynchronize.argtypes = [vp]
+    rt.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
+    rt.hipEventDestroy.argtypes = [vp]
+    nb = gib * 2 ** 30
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    ok = rt.hipMalloc(C.byref(a), nb) == 0 and rt.hipMalloc(C.byref(b), nb) == 0
+    ok = ok and rt.hipEventCreate(C.byref(e0)) == 0 and rt.hipEventCreate(C.byref(e1)) == 0
+    ms = C.c_float(0.0)
+    if ok:
+        rt.hipMemcpy(b, a, nb, 3)  # device to device
+        rt.hipEventRecord(e0, None)
+        for _ in range(reps):
+            rt.hipMemcpy(b, a, nb, 3)
+        rt.hipEventRecord(e1, None)
+        rt.hipEventSynchronize(e1)
+        rt.hipEventElapsedTime(C.byref(ms), e0, e1)
+    for p in (a, b):
+        if p.value:
+            rt.hipFree(p)
+    for e in (e0, e1):
+        if e.value:
+            rt.hipEventDestroy(e)
+    if not ok or ms.value <= 0:
+        return None
+    return 2 * nb / (ms.value / reps * 1e-3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -346,8 +392,9 @@ def main():
     zseg = a.zseg
     if zseg <= 0:
         t = time.time()
-        zseg = ctx.autotune(4)
-        log(f"[bench] autotune: {zseg} rows per wave ({time.time() - t:.1f}s)")
+        zseg = ctx.autotune(8)
+        log(f"[bench] autotune (forward, reconstruction, receiver step): {zseg} rows per wave "
+            f"({time.time() - t:.1f}s)")
     log(f"[bench] setup + modelling {time.time() - t_setup:.1f}s")
 
     def step():
@@ -368,6 +415,8 @@ def main():
         log(f"[bench] step {i + 1}/{a.steps} done at {time.perf_counter() - t0:.2f}s")
     ctx.sync()
     el = time.perf_counter() - t0
+    copy_gbs = copy_bandwidth()
+    log(f"[bench] device copy right after the timed steps: {copy_gbs or 0:.0f} GB/s")
     img, _ = ctx.stack_finish(a.shots * (a.warmup + a.steps), 1)
     assert np.isfinite(img).all(), "non-finite image"
 
@@ -410,7 +459,9 @@ def main():
                      "frac": round(kd["gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": pmc_traffic(dom, f"rvsp_faulted_351x1701_{a.shots}shots"),
                      "algorithmic_bytes_per_launch": kd["bytes_per_launch"],
-                     "avg_launch_ms": round(kd["avg_ms"], 5)},
+                     "avg_launch_ms": round(kd["avg_ms"], 5),
+                     "copy_GBps_same_run": round(copy_gbs, 1) if copy_gbs else None,
+                     "frac_of_copy": round(kd["gbs"] / copy_gbs, 4) if copy_gbs else None},
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "launches": v["launches"],
                         "GB/s": round(v["gbs"], 1)} for k, v in kinds.items()},
     }

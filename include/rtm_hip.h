@@ -216,13 +216,16 @@ int rtm_data_dot(rtm_ctx* ctx, int x, int y, int shot0, int nshot, double* out);
  * residual update r -= alpha*q). */
 int rtm_data_axpy(rtm_ctx* ctx, float a, int shot0, int nshot);
 
-/* Autotuner (not in the reference): times nsteps forward steps of the
- * current batch for each candidate height of the interior kernels' row
- * segments (the kernels round it to a multiple of their register-queue
- * length), keeps the fastest for this context and returns it in *zseg.
- * Needs model, wavelet and shots; leaves the forward pass to be begun anew.
- * Rows per wave trade the vertical halo re-read (2*NFD rows per segment)
- * against the number of waves in flight. */
+/* Autotuner (not in the reference): for each interior kernel of the
+ * migration (forward, source reconstruction, receiver step + imaging) times
+ * nsteps launches of that kernel on the current batch for each candidate
+ * height of its row segments (the kernels round it to a multiple of their
+ * register-queue length) and keeps the fastest per kernel for this context
+ * (Born modelling and the adjoint take the receiver step's); zseg[3], when
+ * not NULL, receives the three choices.  Needs model, wavelet and shots;
+ * leaves the passes to be begun anew.  Rows per wave trade the vertical
+ * halo re-read (2*NFD rows per segment) against the number of waves in
+ * flight. */
 int rtm_autotune(rtm_ctx* ctx, int nsteps, int* zseg);
 
 /* Kernel timing with HIP events on the context stream: mask is a bit set of

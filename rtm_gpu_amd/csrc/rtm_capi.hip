@@ -80,6 +80,7 @@ struct rtm_ctx {
   double* d_dot = nullptr;      // FP64 dot-product scratch
   bool model_ok = false, wav_ok = false, shots_ok = false, data_ok = false, save_strips = true;
   int zs = 32, nxt = 1;
+  int zs_epi[8] = {0};  // rows per wave per interior kernel kind (0: zs)
   // captured migration (hipGraph)
   bool graph_on = false;
   hipGraph_t graph = nullptr;
@@ -202,6 +203,11 @@ void fill_int_args(rtm_ctx* c, IntArgs& a) {
   a.ua = a.ub = a.uc = nullptr;
   a.image = 0;
 }
+// Rows per wave of one interior kernel kind (autotuned per kind, else zs).
+int zs_for(const rtm_ctx* c, int epi) {
+  return (epi >= 0 && epi < 8 && c->zs_epi[epi] > 0) ? c->zs_epi[epi] : c->zs;
+}
+
 void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
   a.g = c->g;
   a.m = c->m;
@@ -227,6 +233,7 @@ int forward_step(rtm_ctx* c, int k) {
     a.wav = wv; a.k = k;
     Timed t(c, RTM_K_FWD_INTERIOR);
     // Add (FP32 combine) or Add_Con (FP64 combine) (kernel.cu:46-114)
+    a.zs = zs_for(c, c->p.ilste ? EPI_FWD64 : EPI_FWD);
     HIPCHK(c, launch_interior(c->p.ilste ? EPI_FWD64 : EPI_FWD, c->nfd, a, c->nshot, c->stream));
   }
   {
@@ -255,6 +262,7 @@ int backward_step(rtm_ctx* c, int k) {
     a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
     a.wav = wv; a.k = k;
     Timed t(c, RTM_K_SRC_INTERIOR);
+    a.zs = zs_for(c, EPI_SRC);
     HIPCHK(c, launch_interior(EPI_SRC, c->nfd, a, c->nshot, c->stream));
   }
   {  // strips of slot k into the rings of u(k): BKEqual of step k-1 (kernel.cu:892)
@@ -271,6 +279,7 @@ int backward_step(rtm_ctx* c, int k) {
     a.uk = c->F[c->f1];
     a.wav = 0.f; a.k = k;
     Timed t(c, RTM_K_RCV_INTERIOR);
+    a.zs = zs_for(c, EPI_RCV);
     HIPCHK(c, launch_interior(EPI_RCV, c->nfd, a, c->nshot, c->stream));
   }
   {  // BKAdd on the frame + BKHybrid1/2 (kernel.cu:913-920)
@@ -298,6 +307,7 @@ int born_step(rtm_ctx* c, int k) {
     a.ua = c->F[c->f1]; a.ub = c->F[c->f0]; a.uc = c->F[c->f2];
     a.wav = 0.f; a.k = k;
     Timed t(c, RTM_K_BORN_INTERIOR);
+    a.zs = zs_for(c, c->p.ilste ? EPI_BORN64 : EPI_BORN);
     HIPCHK(c, launch_interior(c->p.ilste ? EPI_BORN64 : EPI_BORN, c->nfd, a, c->nshot, c->stream));
   }
   {
@@ -326,6 +336,7 @@ int adjoint_update(rtm_ctx* c, int k, bool image) {
     a.image = image ? 1 : 0;
     a.wav = 0.f; a.k = k;
     Timed t(c, RTM_K_ADJ_INTERIOR);
+    a.zs = zs_for(c, EPI_ADJ);
     HIPCHK(c, launch_interior(EPI_ADJ, c->nfd, a, c->nshot, c->stream));
   }
   {
@@ -350,6 +361,7 @@ int adjoint_step(rtm_ctx* c, int k) {
     a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
     a.wav = wv; a.k = k;
     Timed t(c, RTM_K_SRC_INTERIOR);
+    a.zs = zs_for(c, EPI_SRC);
     HIPCHK(c, launch_interior(EPI_SRC, c->nfd, a, c->nshot, c->stream));
   }
   {
@@ -1046,33 +1058,52 @@ int rtm_autotune(rtm_ctx* c, int nsteps, int* zseg) {
   }
   const int saved_timing = c->timing;
   c->timing = 0;
-  int best = c->zs, rc = RTM_OK;
-  float best_ms = 3.0e38f;
-  const int steps = std::min(nsteps, c->p.nt - 3);
-  for (int z : kCand) {
-    c->zs = z;
-    rc = rtm_forward_begin(c, 0);
-    if (rc == RTM_OK) rc = forward_step(c, 2);  // first launch of this shape: not timed
-    if (rc != RTM_OK) break;
-    hipError_t e = hipEventRecord(e0, c->stream);
-    for (int i = 0; i < steps && rc == RTM_OK; ++i) rc = forward_step(c, 3 + i);
-    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e != hipSuccess) rc = fail(c, RTM_E_HIP, "autotune timing: %s", hipGetErrorString(e));
-    if (rc != RTM_OK) break;
-    if (ms < best_ms) { best_ms = ms; best = z; }
+  int rc = rtm_forward_begin(c, 0);  // defined fields to run the kernels on
+  // Each interior kernel kind of the migration is timed alone over nsteps
+  // launches per candidate on the batch's own buffers; the fields it writes
+  // are scratch here (the passes are begun anew afterwards).  Born and the
+  // adjoint share the receiver step's register-queue shape and take its
+  // choice.
+  const int fwd = c->p.ilste ? EPI_FWD64 : EPI_FWD;
+  const int kinds[3] = {fwd, EPI_SRC, EPI_RCV};
+  for (int ki = 0; ki < 3 && rc == RTM_OK; ++ki) {
+    const int epi = kinds[ki];
+    int best = c->zs;
+    float best_ms = 3.0e38f;
+    for (int z : kCand) {
+      IntArgs a;
+      fill_int_args(c, a);
+      const bool rcv = epi == EPI_RCV;
+      a.u1 = rcv ? c->Q[1] : c->F[1]; a.u0 = rcv ? c->Q[0] : c->F[0];
+      a.u2 = rcv ? c->Q[2] : c->F[2];
+      a.uk = c->F[1];
+      a.wav = 0.f; a.k = 2; a.zs = z;
+      hipError_t e = launch_interior(epi, c->nfd, a, c->nshot, c->stream);  // untimed first
+      if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+      for (int i = 0; i < nsteps && e == hipSuccess; ++i)
+        e = launch_interior(epi, c->nfd, a, c->nshot, c->stream);
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) { rc = fail(c, RTM_E_HIP, "autotune timing: %s", hipGetErrorString(e)); break; }
+      if (ms < best_ms) { best_ms = ms; best = z; }
+    }
+    c->zs_epi[epi] = best;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->timing = saved_timing;
-  c->zs = best;
-  c->p.zseg = best;  // kept across later batches of this context
+  c->zs_epi[EPI_BORN] = c->zs_epi[EPI_BORN64] = c->zs_epi[EPI_ADJ] = c->zs_epi[EPI_RCV];
+  c->zs = c->zs_epi[fwd];
   c->kf = -1;
   c->kb = -2;
   drop_graph(c);
-  if (zseg) *zseg = best;
+  if (zseg) {
+    zseg[0] = c->zs_epi[fwd];
+    zseg[1] = c->zs_epi[EPI_SRC];
+    zseg[2] = c->zs_epi[EPI_RCV];
+  }
   return rc;
 }
 

@@ -262,16 +262,20 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   const int lane = threadIdx.x & 63;
   // wave index: uniform in the wave, so row offsets derived from it stay scalar
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // Block b covers shot b % nshot of tile b / nshot.  Workgroups reach the
-  // eight XCDs round-robin by b, so the blocks one XCD holds at a time are
-  // many shots of the same few tiles, and their reads of the shared model
-  // rows (p, coefficient offsets, coefficient rows) hit that XCD's L2.
-  const int shot = (int)(blockIdx.x % (unsigned)a.nshot);
-  const int tile = (int)(blockIdx.x / (unsigned)a.nshot);
+  // The four waves of block b march the same rows and columns (tile
+  // b / nsg) for four shots (group b % nsg), so their reads of the shared
+  // model rows (p, coefficient offsets, coefficient rows) coincide in time and
+  // hit the CU's L1.  Workgroups reach the eight XCDs round-robin by b, so
+  // the blocks one XCD holds at a time are more shots of the same few tiles,
+  // whose model reads hit that XCD's L2.
+  const unsigned nsg = ((unsigned)a.nshot + 3u) >> 2;
+  const int shot = (int)(blockIdx.x % nsg) * 4 + wv;
+  const int tile = (int)(blockIdx.x / nsg);
+  if (shot >= a.nshot) return;
   const int xt = tile % a.nxt, zg = tile / a.nxt;
   const ShotGeo sg = a.geo[shot];
   const int zhi = g.nz - g.n2;
-  const int zs = g.n2 + (zg * 4 + wv) * a.zs;  // a.zs is a multiple of QL
+  const int zs = g.n2 + zg * a.zs;  // a.zs is a multiple of QL
   const int cend = sg.nx - g.n2;                // first column past the shot's interior
   const int cb = g.n2 + xt * 256;
   if (zs >= zhi || cb >= cend) return;          // tiles past a narrower shot's width
@@ -994,7 +998,7 @@ hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s
   a.zs = mult * ql;
   const int nseg = (a.g.mnz + a.zs - 1) / a.zs;
   a.nshot = nshot;
-  dim3 grid(a.nxt * ((nseg + 3) / 4) * nshot), block(256);
+  dim3 grid(a.nxt * nseg * ((nshot + 3) / 4)), block(256);
 #define CASE(N)                                                                             \
   case N:                                                                                   \
     if (epi == EPI_FWD) hipLaunchKernelGGL((k_interior<N, EPI_FWD>), grid, block, 0, s, a);     \

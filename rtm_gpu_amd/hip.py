@@ -279,11 +279,12 @@ class Context:
         """gathers += a * records on the device, over shots [shot0, shot0 + nshot)."""
         self._chk(lib().rtm_data_axpy(self.h, float(a), int(shot0), int(nshot)), "rtm_data_axpy")
 
-    def autotune(self, nsteps=4):
-        """Pick the interior row-segment height by timing forward steps; returns it."""
-        z = C.c_int()
-        self._chk(lib().rtm_autotune(self.h, int(nsteps), C.byref(z)), "rtm_autotune")
-        return z.value
+    def autotune(self, nsteps=8):
+        """Pick the row-segment height of each interior kernel (forward, source
+        reconstruction, receiver step) by timing its launches; returns the three."""
+        z = (C.c_int * 3)()
+        self._chk(lib().rtm_autotune(self.h, int(nsteps), z), "rtm_autotune")
+        return list(z)
 
     def set_timing(self, mask):
         """mask: bit set of kernel kinds (1 << K_*), 0 off, -1 all."""
