@@ -290,6 +290,31 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
 
 
+def run_deblur():
+    """Time to an image: the network applied to the migrated image against 10
+    CGLS iterations, on the faulted surface survey (rtm_gpu_amd.deblur)."""
+    from rtm_gpu_amd import deblur
+    res, _ = deblur.run()
+    out = {"metric": "time to image, deblurring network vs 10-iteration CGLS (speed-up)",
+           "value": round(res["speedup_vs_cg"], 3), "unit": "x", "n_gpus": 1, "steps": 1,
+           "warmup": 0, "ms_per_step": round((res["t_image_s"] + res["t_net_s"]) * 1e3, 3),
+           "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+           "dtype": "f32 stencil, bf16 network (fp32 accumulation), fp64 CG products",
+           "data": "synthetic (seeded faulted model; Born data of its reflectivity)",
+           "config": {"workload": f"faulted_surface_{res['grid']}_{res['shots']}shots_nt{res['nt']}",
+                      "cg_iterations": 10, "training_pairs": res["npairs"],
+                      "epochs": res["epochs"]},
+           "rel_l2_network_vs_cgls": round(res["rel_net_vs_cg"], 4),
+           "rel_l2_scaled_image_vs_cgls": round(res["rel_rtm_vs_cg"], 4),
+           "data_residual": {"scaled_image": round(res["resid_rtm"], 4),
+                             "network": round(res["resid_net"], 4),
+                             "cgls": round(res["resid_cg"], 4)},
+           "times_s": {"image": round(res["t_image_s"], 4), "network": round(res["t_net_s"], 5),
+                       "cgls": round(res["t_cg_s"], 3), "training_pairs": round(res["pairs_s"], 1),
+                       "training": round(res["train_s"], 1)}}
+    print(json.dumps(out), flush=True)
+
+
 def copy_bandwidth(gib=2, reps=5):
     """Device-to-device copy rate (read + write bytes / time) of a gib-GiB
     buffer, timed with HIP events: the bandwidth this GPU delivers to a plain
@@ -346,9 +371,11 @@ def main():
     ap.add_argument("--strips", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--zseg", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="rvsp", choices=["rvsp", "apertures"],
+    ap.add_argument("--workload", default="rvsp", choices=["rvsp", "apertures", "deblur"],
                     help="rvsp: the headline reverse-VSP run; apertures: uneven-aperture "
-                         "surface shots in ragged batches vs one shot per launch")
+                         "surface shots in ragged batches vs one shot per launch; deblur: the "
+                         "deblurring network against 10-iteration CGLS least-squares migration "
+                         "(training pairs, training and evaluation, about 4 minutes)")
     ap.add_argument("--batch", type=int, default=128, help="apertures: shots per launch")
     ap.add_argument("--sample", type=int, default=8,
                     help="apertures: shots timed one per launch")
@@ -361,6 +388,9 @@ def main():
               "GPUs with one image all-reduce at the end, which this build does not run.",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "deblur":
+        run_deblur()
+        return
     if a.workload == "apertures":
         if a.shots == 128 and "--shots" not in sys.argv:
             a.shots = 256
