@@ -130,6 +130,19 @@ template <typename T>
 int dalloc_t(rtm_ctx* c, T** p, size_t count) {
   return dalloc(c, reinterpret_cast<void**>(p), count * sizeof(T));
 }
+// Release one allocation made by dalloc before the context is destroyed.
+template <typename T>
+void dfree_t(rtm_ctx* c, T** p, size_t count) {
+  if (!*p) return;
+  for (size_t i = 0; i < c->allocs.size(); ++i)
+    if (c->allocs[i] == static_cast<void*>(*p)) {
+      c->allocs.erase(c->allocs.begin() + i);
+      c->bytes -= count * sizeof(T);
+      break;
+    }
+  (void)hipFree(*p);
+  *p = nullptr;
+}
 
 // IEEE binary16 -> float on the host.
 float half_to_float(unsigned short h) {
@@ -873,7 +886,10 @@ int rtm_born_begin(rtm_ctx* c) {
   for (int s = 0; s < c->nshot; ++s) uniform = uniform && c->geo[s].nx == g.nx && c->geo[s].x_img == 0;
   const long long rstride = uniform ? 0 : g.fstride;
   const size_t need = (size_t)(uniform ? 1 : c->nshot) * g.fstride;
-  if (c->refl_cap < need) {
+  if (c->refl_cap < need) {  // a ragged batch needs one field per shot: replace the shared one
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree_t(c, &c->d_refl, c->refl_cap);
+    c->refl_cap = 0;
     const size_t cap = uniform ? need : (size_t)c->p.max_shots * g.fstride;
     rc = dalloc_t(c, &c->d_refl, cap);
     if (rc != RTM_OK) return rc;

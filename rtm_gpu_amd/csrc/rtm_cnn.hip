@@ -333,6 +333,19 @@ int calloc_dev(rtm_cnn* c, T** p, size_t count) {
   return RTM_CNN_OK;
 }
 
+// Release one allocation made by calloc_dev before the network is destroyed.
+template <typename T>
+void cfree_dev(rtm_cnn* c, T** p) {
+  if (!*p) return;
+  for (size_t i = 0; i < c->allocs.size(); ++i)
+    if (c->allocs[i] == static_cast<void*>(*p)) {
+      c->allocs.erase(c->allocs.begin() + i);
+      break;
+    }
+  (void)hipFree(*p);
+  *p = nullptr;
+}
+
 int check_shape(rtm_cnn* c, int n, int h, int w) {
   if (n < 1 || h < 1 || w < 1) return cfail(c, RTM_CNN_E_ARG, "shape %d x %d x %d", n, h, w);
   if ((long long)n * h * w > c->cap_pix) return cfail(c, RTM_CNN_E_ARG, "%lld pixels > capacity %lld", (long long)n * h * w, c->cap_pix);
@@ -596,7 +609,11 @@ int rtm_cnn_fit(rtm_cnn* c, const float* x, const float* t, int n, int h, int w,
   if (rc != RTM_CNN_OK) return rc;
   const size_t img = (size_t)h * w;
   const size_t need = (size_t)n * img;
-  if ((long long)need > c->ds_cap) {
+  if ((long long)need > c->ds_cap) {  // a larger training set replaces the old copy
+    CNNCHK(c, hipStreamSynchronize(c->stream));
+    cfree_dev(c, &c->ds_x);
+    cfree_dev(c, &c->ds_t);
+    c->ds_cap = 0;
     rc = calloc_dev(c, &c->ds_x, need * c->c[0]);
     if (rc == RTM_CNN_OK) rc = calloc_dev(c, &c->ds_t, need);
     if (rc != RTM_CNN_OK) return rc;

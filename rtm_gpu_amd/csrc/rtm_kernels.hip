@@ -359,14 +359,7 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
     u0b[d] = bld4(R0, vcol, rowb(zs + d));
     hvb[d] = bld4(R1, vhal, rowb(zs + d));
   }
-  float4 ukb[2], i1b[2], i2b[2], sub[2], sqb[2];
-  if (T::rcv) {
-    ukb[0] = bld4(RU, vcol, rowb(zs));
-    i1b[0] = bld4(RI1, vcol, rowb(zs));
-    i2b[0] = bld4(RI2, vcol, rowb(zs));
-    sub[0] = bld4(RSU, vcs, rowb(zs));
-    sqb[0] = bld4(RSQ, vcs, rowb(zs));
-  }
+  float4 ukb[1], i1b[1], i2b[1], sub[1], sqb[1];
   float4 xab[2], xbb[2], xcb[2], xxb[2];
   if (T::born || T::adj) {
     xab[0] = bld4(RA, vxl, rowb(zs));
@@ -386,19 +379,23 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
       cr[2].load(RC, kb[0].z);
       cr[3].load(RC, kb[0].w);
       __builtin_amdgcn_sched_barrier(0);
+      // Receiver step: this row's imaging operands, issued ahead of the
+      // prefetches so that their wait leaves the prefetches in flight (a
+      // row of look-ahead for them would cost 20 VGPRs the kernel needs).
+      if (T::rcv) {
+        const unsigned rzz = rowb(zz);
+        ukb[0] = bld4(RU, vcol, rzz);
+        i1b[0] = bld4(RI1, vcol, rzz);
+        i2b[0] = bld4(RI2, vcol, rzz);
+        sub[0] = bld4(RSU, vcs, rzz);
+        sqb[0] = bld4(RSQ, vcs, rzz);
+      }
       // (b) operands of later rows
       q[(j + 2 * NFD + PD) % QL] = bld4(R1, vcol, rowb(zz + NFD + PD));
       kb[PD] = bld4u(RK, vcol, rowb(zz + PD));
       pb[PD] = bld4(RP, vcol, rowb(zz + PD));
       u0b[PD] = bld4(R0, vcol, rowb(zz + PD));
       hvb[PD] = bld4(R1, vhal, rowb(zz + PD));
-      if (T::rcv) {
-        ukb[1] = bld4(RU, vcol, rowb(zz + 1));
-        i1b[1] = bld4(RI1, vcol, rowb(zz + 1));
-        i2b[1] = bld4(RI2, vcol, rowb(zz + 1));
-        sub[1] = bld4(RSU, vcs, rowb(zz + 1));
-        sqb[1] = bld4(RSQ, vcs, rowb(zz + 1));
-      }
       if (T::born || T::adj) {
         xab[1] = bld4(RA, vxl, rowb(zz + 1));
         xbb[1] = bld4(RB, vxl, rowb(zz + 1));
@@ -462,7 +459,6 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
         bst4(RI2, vst | rowdrop, rz, r2o[0], r2o[1], r2o[2], r2o[3]);
         bst4(RSU, vss | rowdrop, rz, rsu[0], rsu[1], rsu[2], rsu[3]);
         bst4(RSQ, vss | rowdrop, rz, rsq[0], rsq[1], rsq[2], rsq[3]);
-        ukb[0] = ukb[1]; i1b[0] = i1b[1]; i2b[0] = i2b[1]; sub[0] = sub[1]; sqb[0] = sqb[1];
       }
       if (T::adj) {  // G += D2u(k+1) * mu(k+2); mu(k+2) is this update's u0
         float go[4];

@@ -83,3 +83,25 @@ def test_grouped_cgls_equals_independent_runs():
         for a, b in zip(hist, hg):
             assert a["rr"][g] == b["rr"]
     assert not m[2].any()
+
+
+def test_deblur_helpers():
+    """Scattering strength, network input channels and seeded training
+    reflectivities of the deblurring pipeline (host code)."""
+    from rtm_gpu_amd import deblur
+    raw = np.array([[2000.0, 2500.0], [3000.0, 1500.0]], np.float32)  # [x][z]
+    sm = np.full_like(raw, 2000.0)
+    m = deblur.reflectivity(raw, sm)
+    assert m.shape == (2, 2)  # [z][x]
+    assert np.allclose(m, [[0.0, 1.25], [0.5625, -0.4375]])
+    S = deblur.build_survey(mod_nz=24, mod_nx=40, nt=200, nshot=2, nfdmax=4, n2=10)
+    a = deblur.random_reflectivity(S, 7)
+    assert a.shape == (24, 40) and np.array_equal(a, deblur.random_reflectivity(S, 7))
+    assert not np.array_equal(a, deblur.random_reflectivity(S, 8))
+    illum = np.outer(np.linspace(1.0, 1e-8, 24), np.ones(40))
+    x = deblur.features(S, np.ones((3, 24, 40)), 0.5, illum)
+    assert x.shape == (3, 3, 24, 40)
+    assert np.all(x[:, 0] == 0.5)
+    assert x[0, 1].max() == 1.0 and x[0, 1].min() == 0.0  # log illumination, clipped at 1e-6
+    assert x[0, 2, 0, 0] == 0.0 and x[0, 2, -1, 0] == 1.0  # depth
+    assert S["r_x"] == sorted(S["r_x"]) and len(set(S["r_x"])) == 2
