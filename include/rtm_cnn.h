@@ -26,7 +26,7 @@ enum { RTM_CNN_OK = 0, RTM_CNN_E_ARG = -1, RTM_CNN_E_HIP = -2, RTM_CNN_E_NOMEM =
 typedef struct rtm_cnn rtm_cnn;
 
 /* channels[0..nlayer]: input channels, hidden widths, 1 output channel; every
- * width 1..16, nlayer 1..16.  dilation[0..nlayer-1]: per-layer dilation 1..64
+ * width 1..32, nlayer 1..16.  dilation[0..nlayer-1]: per-layer dilation 1..64
  * of the 3x3 taps (NULL: all 1; zero padding of the same width).  max_pixels
  * bounds n*h*w of a batch, max_rows bounds n*h.  Parameters start at zero
  * (the network then passes x[0]). */

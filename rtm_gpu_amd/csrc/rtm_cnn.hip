@@ -19,9 +19,12 @@
 //                the last layer y = z + x0 in FP32
 //   input grad   dz_prev = conv(dz, W flipped and transposed) * (a > 0)
 //   weight grad  dW[tap][co][ci] = sum_pix dz[pix][co] * a[pix + tap][ci]:
-//                M = co, N = ci, K = pixels; one wave per image row writes
-//                its partial sums, a second kernel adds them in row order
-//                (FP64), so the result does not depend on scheduling.
+//                M = co, N = ci, K = pixels; one wave per image row and
+//                16 x 16 channel block writes its partial sums, a second
+//                kernel adds them in row order (FP64), so the result does not
+//                depend on scheduling.
+// Channel counts above 16 run in blocks of 16 output channels (one wave per
+// block) and, for the weight gradient, blocks of 16 input channels.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
@@ -40,7 +43,7 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
-constexpr int MAXC = 16;                    // channels per layer
+constexpr int MAXC = 32;                    // channels per layer (two blocks of 16)
 constexpr int MAXL = 16;                    // layers
 constexpr int WG_STRIDE = 9 * MAXC * MAXC + MAXC;  // weight-grad partials per row
 
@@ -70,21 +73,25 @@ struct ConvArgs {
   const float* resid; // [n][h][w] (mode 1)
   const bf16* mask;   // NHWC, cout_p channels (mode 2)
   int n, h, w, cin_p, kpad, cout_p, mode;
+  int ncob;           // blocks of 16 output channels
   int dil;            // dilation: tap (ky, kx) reads (y + (ky-1)*dil, x + (kx-1)*dil)
 };
 
 __global__ __launch_bounds__(256) void k_conv(const ConvArgs a) {
   const int lane = threadIdx.x & 63;
   const int segs = (a.w + 15) >> 4;
-  const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (wave >= (long long)a.n * a.h * segs) return;
+  const long long wave_all = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long per = (long long)a.n * a.h * segs;
+  if (wave_all >= per * a.ncob) return;
+  const int cob = (int)(wave_all / per);  // block of 16 output channels
+  const long long wave = wave_all - cob * per;
   const int seg = (int)(wave % segs);
   const long long row = wave / segs;  // nn*h + y
   const int y = (int)(row % a.h);
   const long long nn = row / a.h;
   const int m = lane & 15, kg = lane >> 4;
   const int x = seg * 16 + m;
-  const bf16* wrow = a.wt + (long long)m * a.kpad;  // output channel n = lane & 15
+  const bf16* wrow = a.wt + (long long)(cob * 16 + m) * a.kpad;  // output channel of lane & 15
   f4 acc = {0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < a.kpad; k0 += 16) {
     const int k = k0 + 4 * kg;
@@ -98,7 +105,7 @@ __global__ __launch_bounds__(256) void k_conv(const ConvArgs a) {
     const s4 bv = load4(wrow + k);
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
   }
-  const int co = lane & 15;
+  const int co = cob * 16 + (lane & 15);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int px = seg * 16 + 4 * kg + j;
@@ -127,11 +134,17 @@ struct WgradArgs {
 
 __global__ __launch_bounds__(256) void k_wgrad(const WgradArgs a) {
   const int lane = threadIdx.x & 63;
-  const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= (long long)a.n * a.h) return;
+  const long long rows = (long long)a.n * a.h;
+  const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int ncob = (a.dz_p + 15) >> 4, ncib = (a.act_p + 15) >> 4;
+  if (wave >= rows * ncob * ncib) return;
+  const long long row = wave % rows;
+  const int blk = (int)(wave / rows);
+  const int cob = blk % ncob, cib = blk / ncob;  // 16 x 16 block of (co, ci)
   const int y = (int)(row % a.h);
   const long long nn = row / a.h;
   const int c16 = lane & 15, kg = lane >> 4;
+  const int coA = cob * 16 + c16, ciB = cib * 16 + c16;
   f4 acc[9];
 #pragma unroll
   for (int t = 0; t < 9; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
@@ -143,7 +156,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const WgradArgs a) {
     for (int j = 0; j < 4; ++j) {
       const int px = x0 + 4 * kg + j;
       bf16 v = 0;
-      if (px < a.w && c16 < a.dz_p) v = a.dz[((nn * a.h + y) * a.w + px) * a.dz_p + c16];
+      if (px < a.w && coA < a.dz_p) v = a.dz[((nn * a.h + y) * a.w + px) * a.dz_p + coA];
       av[j] = (short)v;
       bsum += from_bf16(v);
     }
@@ -157,8 +170,8 @@ __global__ __launch_bounds__(256) void k_wgrad(const WgradArgs a) {
         const int px = x0 + 4 * kg + j;
         const int xx = px + (t % 3 - 1) * a.dil;
         bf16 v = 0;
-        if (px < a.w && c16 < a.act_p && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
-          v = a.act[((nn * a.h + yy) * a.w + xx) * a.act_p + c16];
+        if (px < a.w && ciB < a.act_p && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
+          v = a.act[((nn * a.h + yy) * a.w + xx) * a.act_p + ciB];
         bv[j] = (short)v;
       }
       acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
@@ -169,11 +182,11 @@ __global__ __launch_bounds__(256) void k_wgrad(const WgradArgs a) {
   for (int t = 0; t < 9; ++t)
 #pragma unroll
     for (int j = 0; j < 4; ++j)  // D[m = co][n = ci]
-      P[(t * MAXC + 4 * kg + j) * MAXC + c16] = acc[t][j];
+      P[(t * MAXC + cob * 16 + 4 * kg + j) * MAXC + ciB] = acc[t][j];
   // bias: lanes c16, c16+16, c16+32, c16+48 hold pixels of the same channel
   bsum += __shfl_xor(bsum, 16);
   bsum += __shfl_xor(bsum, 32);
-  if (kg == 0) P[9 * MAXC * MAXC + c16] = bsum;
+  if (kg == 0 && cib == 0) P[9 * MAXC * MAXC + coA] = bsum;
 }
 
 // grad[layer params] from the row partials: W[co][ci][ky][kx] then b[co].
@@ -365,7 +378,7 @@ int pack_weights(rtm_cnn* c) {
 }
 
 int launch_conv(rtm_cnn* c, const ConvArgs& a) {
-  const long long waves = (long long)a.n * a.h * ((a.w + 15) / 16);
+  const long long waves = (long long)a.n * a.h * ((a.w + 15) / 16) * a.ncob;
   hipLaunchKernelGGL(k_conv, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, c->stream, a);
   CNNCHK(c, hipGetLastError());
   return RTM_CNN_OK;
@@ -383,6 +396,7 @@ int forward_dev(rtm_cnn* c, int n, int h, int w) {
     a.kpad = c->kf[l];
     a.cout_p = c->cp[l + 1];
     a.dil = c->dil[l];
+    a.ncob = (l + 1 < c->L) ? (c->cp[l + 1] + 15) / 16 : 1;
     if (l + 1 < c->L) {
       a.mode = 0;
       a.out = c->act[l + 1];
@@ -417,7 +431,8 @@ int loss_grad_dev(rtm_cnn* c, const float* t_dev, int n, int h, int w, double* l
   for (int l = L - 1; l >= 0; --l) {
     WgradArgs g{c->dz[l], c->act[l], c->part, n, h, w, c->cp[l + 1], c->cp[l], c->dil[l]};
     const long long rows = (long long)n * h;
-    hipLaunchKernelGGL(k_wgrad, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, c->stream, g);
+    const long long nwave = rows * ((c->cp[l + 1] + 15) / 16) * ((c->cp[l] + 15) / 16);
+    hipLaunchKernelGGL(k_wgrad, dim3((unsigned)((nwave + 3) / 4)), dim3(256), 0, c->stream, g);
     CNNCHK(c, hipGetLastError());
     const int ne = c->c[l + 1] * c->c[l] * 9 + c->c[l + 1];
     hipLaunchKernelGGL(k_wgrad_reduce, dim3((ne + 63) / 64), dim3(64), 0, c->stream, c->part, rows,
@@ -435,6 +450,7 @@ int loss_grad_dev(rtm_cnn* c, const float* t_dev, int n, int h, int w, double* l
       a.cout_p = c->cp[l];
       a.mode = 2;
       a.dil = c->dil[l];
+      a.ncob = (c->cp[l] + 15) / 16;
       rc = launch_conv(c, a);
       if (rc != RTM_CNN_OK) return rc;
     }

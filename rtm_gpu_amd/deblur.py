@@ -159,6 +159,107 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
+def patches(XF, M, size, count, rng):
+    """count random size x size crops (all channels) of the pairs."""
+    n, c, hh, ww = XF.shape
+    xs = np.empty((count, c, size, size), np.float32)
+    ms = np.empty((count, size, size), np.float32)
+    k = rng.integers(0, n, count)
+    z = rng.integers(0, hh - size + 1, count)
+    x = rng.integers(0, ww - size + 1, count)
+    for i in range(count):
+        xs[i] = XF[k[i], :, z[i]:z[i] + size, x[i]:x[i] + size]
+        ms[i] = M[k[i], z[i]:z[i] + size, x[i]:x[i] + size]
+    return xs, ms
+
+
+def panel_residuals(ctx, panels, dm):
+    """||L dm_p|| per panel (dm [panels][mnz][mnx]) from one batched Born pass."""
+    op = lsrtm.BornOperator(ctx, groups=panels)
+    return np.sqrt(np.maximum(op.forward(dm), 0.0))
+
+
+def run_models(S, ntrain=96, neval=8, panels=8, epochs=100, patch=64, batch=256,
+               patches_per_epoch=2048, lr=1e-3, channels=(3,) + (32,) * 5 + (1,),
+               dilation=None, cg_iters=10, target="true", seed=0):
+    """Several random models: train on engine-made pairs (patches), evaluate on
+    held-out models against CGLS with per-model statistics."""
+    res = dict(grid=f"{S['mod_nz']}x{S['mod_nx']}", nt=int(S["d"].nt), shots=S["nshot"],
+               ntrain=ntrain, neval=neval, epochs=epochs, patch=patch, batch=batch,
+               channels=list(channels), dilation=list(dilation) if dilation else None,
+               target=target)
+    t = time.time()
+    X, M = make_pairs(S, ntrain, panels, seed0=1000, target=target, cg_iters=cg_iters)
+    res["pairs_s"] = time.time() - t
+    log(f"[deblur] {ntrain} training pairs in {res['pairs_s']:.1f}s")
+    c = float(np.sum(X.astype(np.float64) * M) / np.sum(X.astype(np.float64) ** 2))
+    res["scale"] = c
+    illum = illumination(S)
+    XF = features(S, X, c, illum)
+    del X
+    rng = np.random.default_rng(seed)
+    net = cnn.Deblur(list(channels), batch * patch * patch, batch * patch, dilation=dilation)
+    net.init_params(seed=seed)
+    t = time.time()
+    hist = []
+    for ep in range(epochs):  # fresh random crops each epoch
+        xp, mp = patches(XF, M, patch, patches_per_epoch, rng)
+        hist.extend(net.fit(xp, mp, epochs=1, batch=batch, lr=lr, seed=seed + ep))
+        if (ep + 1) % 10 == 0:
+            log(f"[deblur] epoch {ep + 1} of {epochs}: loss {hist[-1]:.3e}")
+    res["train_s"] = time.time() - t
+    res["loss_first"], res["loss_last"] = float(hist[0]), float(hist[-1])
+    del XF, M
+    # held-out models: image, network, CGLS per panel
+    ctx = context(S, panels)
+    per = []
+    for b in range(0, neval, panels):
+        k = min(panels, neval - b)
+        mt = np.stack([random_reflectivity(S, 5000 + b + i) for i in range(panels)])
+        if k < panels:
+            mt[k:] = 0.0
+        normal_images(ctx, mt)  # Born data d of m into the gathers
+        t = time.time()  # the migrated image of d: a forward pass and the adjoint pass
+        ctx.forward_begin(True)
+        ctx.forward_advance(ctx.p.nt)
+        ctx.adjoint_begin()
+        ctx.adjoint_advance(ctx.p.nt)
+        g = ctx.adjoint_gradient()
+        img = np.ascontiguousarray(g.reshape(g.shape[0], panels, -1).transpose(1, 0, 2))
+        t_img = time.time() - t
+        xe = features(S, img, c, illum)
+        t = time.time()
+        m_net = np.stack([net.forward(xe[i:i + 1])[0] for i in range(panels)])
+        t_net = time.time() - t
+        t = time.time()
+        m_cg, _ = lsrtm.cgls(lsrtm.BornOperator(ctx, groups=panels), None, cg_iters,
+                             callback=lambda it, *_: log(f"[deblur] held-out CGLS iteration {it}"))
+        t_cg = time.time() - t
+        m_rtm = np.float32(c) * img
+        dn = panel_residuals(ctx, panels, mt)
+        rn = {name: panel_residuals(ctx, panels, est - mt) / np.where(dn > 0, dn, 1.0)
+              for name, est in (("rtm", m_rtm), ("net", m_net), ("cg", m_cg))}
+        for i in range(k):
+            per.append(dict(rel_net_vs_cg=rel(m_net[i], m_cg[i]), rel_rtm_vs_cg=rel(m_rtm[i], m_cg[i]),
+                            rel_net_vs_true=rel(m_net[i], mt[i]), rel_cg_vs_true=rel(m_cg[i], mt[i]),
+                            resid_rtm=float(rn["rtm"][i]), resid_net=float(rn["net"][i]),
+                            resid_cg=float(rn["cg"][i])))
+        res.setdefault("t_image_s", t_img)
+        res.setdefault("t_net_s", t_net)
+        res.setdefault("t_cg_s", t_cg)
+        log(f"[deblur] held-out models {b}..{b + k - 1}: rel to CGLS net "
+            f"{np.mean([q['rel_net_vs_cg'] for q in per[-k:]]):.3f}")
+    for key in per[0]:
+        res[key] = float(np.mean([q[key] for q in per]))
+    res["speedup_vs_cg"] = res["t_cg_s"] / (res["t_image_s"] + res["t_net_s"])
+    res["per_model"] = per
+    log(f"[deblur] {neval} held-out models: rel L2 to CGLS net {res['rel_net_vs_cg']:.3f}, "
+        f"scaled image {res['rel_rtm_vs_cg']:.3f}; data residual RTM {res['resid_rtm']:.3f}, "
+        f"net {res['resid_net']:.3f}, CGLS {res['resid_cg']:.3f}; time image+net "
+        f"{res['t_image_s'] + res['t_net_s']:.2f}s vs CGLS {res['t_cg_s']:.2f}s")
+    return res
+
+
 def run(S=None, npairs=128, panels=32, epochs=600, batch=8, lr=1e-3,
         channels=(3,) + (16,) * 11 + (1,), dilation=(1, 2, 4, 8, 16, 32, 16, 8, 4, 2, 1, 1),
         cg_iters=10, seed=0, target="cg"):
@@ -245,7 +346,20 @@ def main():
     ap.add_argument("--target", default="cg", choices=["cg", "true"],
                     help="train toward the CGLS image of each pair or its reflectivity")
     ap.add_argument("--save", default="", help="write the four images to this .npz")
+    ap.add_argument("--models", action="store_true",
+                    help="many random models (1024 x 256, 64 shots, 25 Hz, 3000 steps; 6 x 32-"
+                         "channel layers on 64 x 64 patches, batch 256), held-out evaluation")
+    ap.add_argument("--ntrain", type=int, default=96)
+    ap.add_argument("--neval", type=int, default=8)
     a = ap.parse_args()
+    if a.models:
+        S = build_survey(mod_nz=256, mod_nx=1024, h=10.0, f0=25.0, nt=3000, nshot=64, nfdmax=4,
+                         n2=30)
+        res = run_models(S, ntrain=a.ntrain, neval=a.neval, epochs=a.epochs if a.epochs != 600
+                         else 100, target=a.target if "--target" in __import__("sys").argv
+                         else "true")
+        print(json.dumps(res))
+        return
     S = build_survey(nshot=a.shots, nt=a.nt)
     res, im = run(S, npairs=a.npairs, panels=a.panels, epochs=a.epochs, batch=a.batch, lr=a.lr,
                   cg_iters=a.cg_iters, target=a.target)

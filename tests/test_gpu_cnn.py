@@ -94,7 +94,8 @@ def test_one_layer_forward_and_gradient(ch):
     assert _rel(g, gr) < 1e-4
 
 
-@pytest.mark.parametrize("ch", [[2, 16, 16, 16, 1], [2, 8, 12, 1]])
+@pytest.mark.parametrize("ch", [[2, 16, 16, 16, 1], [2, 8, 12, 1], [3, 32, 32, 32, 1],
+                                [2, 20, 32, 12, 1]])
 def test_deep_forward_and_gradient(ch):
     net, p = _net(ch, seed=5)
     x, t = _data(2, ch[0], 24, 50, seed=4)

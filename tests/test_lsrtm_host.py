@@ -105,3 +105,19 @@ def test_deblur_helpers():
     assert x[0, 1].max() == 1.0 and x[0, 1].min() == 0.0  # log illumination, clipped at 1e-6
     assert x[0, 2, 0, 0] == 0.0 and x[0, 2, -1, 0] == 1.0  # depth
     assert S["r_x"] == sorted(S["r_x"]) and len(set(S["r_x"])) == 2
+
+
+def test_deblur_patches():
+    from rtm_gpu_amd import deblur
+    rng = np.random.default_rng(0)
+    XF = rng.standard_normal((3, 2, 20, 30)).astype(np.float32)
+    M = rng.standard_normal((3, 20, 30)).astype(np.float32)
+    xs, ms = deblur.patches(XF, M, 8, 50, np.random.default_rng(1))
+    assert xs.shape == (50, 2, 8, 8) and ms.shape == (50, 8, 8)
+    # every crop is an aligned window of one pair, inputs and target together
+    for i in range(50):
+        hits = [(k, z, x) for k in range(3) for z in range(13) for x in range(23)
+                if np.array_equal(ms[i], M[k, z:z + 8, x:x + 8])]
+        assert len(hits) == 1
+        k, z, x = hits[0]
+        assert np.array_equal(xs[i], XF[k, :, z:z + 8, x:x + 8])
