@@ -28,6 +28,10 @@ CASES = {
     # maximum operator length of the reference parameter file (nfdmax = N2 = 10)
     "ls_nfd10_n2_10": dict(name="ls_nfd10_n2_10", nfdmax=10, n2=10, mod_nz=36, mod_nx=48, seed=4,
                            h=12.5, hz=12.5, f0=20.0, vhi=3500.0),
+    # the longest operator the kernels take (nfdmax = n2 = 12, Taylor rows), an
+    # interior width that is not a multiple of 4
+    "taylor_nfd12_n2_12": dict(name="taylor_nfd12_n2_12", ilste=1, nfdmax=12, n2=12, mod_nz=38,
+                               mod_nx=53, seed=5, h=12.5, hz=12.5, f0=20.0, src_x1=20),
 }
 
 
