@@ -371,6 +371,9 @@ def main():
     ap.add_argument("--strips", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--zseg", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--host-gathers", action="store_true",
+                    help="every timed step uploads the gathers from host memory (the PCIe-"
+                         "inclusive rate of a caller that hands over host buffers)")
     ap.add_argument("--workload", default="rvsp", choices=["rvsp", "apertures", "deblur"],
                     help="rvsp: the headline reverse-VSP run; apertures: uneven-aperture "
                          "surface shots in ragged batches vs one shot per launch; deblur: the "
@@ -427,7 +430,11 @@ def main():
             f"({time.time() - t:.1f}s)")
     log(f"[bench] setup + modelling {time.time() - t_setup:.1f}s")
 
+    host_seis = ctx.records() if a.host_gathers else None  # [shot][n][nt], trace-major
+
     def step():
+        if host_seis is not None:
+            ctx.set_gathers(host_seis)
         ctx.migrate()
         ctx.image_post(mm["vmax"], cfg.whitecoe, want=False)
 
@@ -477,7 +484,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded faulted-layer model; gathers modelled by the same engine)",
+        "data": "synthetic (seeded faulted-layer model; gathers modelled by the same engine)"
+                + ("; gathers uploaded from host memory in every timed step" if a.host_gathers
+                   else ""),
         "config": {"workload": workload, "grid": [d.nz, d.nx], "interior": [d.nz - 60, d.nx - 60],
                    "nt": d.nt, "shots": a.shots, "nfdmax": cfg.nfdmax, "n2": cfg.n2,
                    "strips": a.strips, "coefficients": "adaptive LS",

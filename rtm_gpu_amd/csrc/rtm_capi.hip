@@ -77,6 +77,7 @@ struct rtm_ctx {
   size_t refl_cap = 0;
   bool refl_ok = false;
   float* d_grad = nullptr;      // [mnz][gw]
+  float* d_stage = nullptr;     // one shot's trace-major gather [n][nt] (host transfers)
   double* d_dot = nullptr;      // FP64 dot-product scratch
   bool model_ok = false, wav_ok = false, shots_ok = false, data_ok = false, save_strips = true;
   int zs = 32, nxt = 1;
@@ -763,18 +764,25 @@ int rtm_set_shot_model(rtm_ctx* c, int shot, const float* v, const float* r1) {
   return RTM_OK;
 }
 
+// One shot's trace-major staging buffer for host transfers of gathers and
+// records (the device layout is time-major); transposed on the GPU.
+static int stage_buffer(rtm_ctx* c) {
+  if (c->d_stage) return RTM_OK;
+  return dalloc_t(c, &c->d_stage, (size_t)c->p.n * c->p.nt);
+}
+
 int rtm_set_gathers(rtm_ctx* c, const float* seis) {
   if (!c || !seis) return RTM_E_ARG;
   if (!c->shots_ok) return fail(c, RTM_E_STATE, "rtm_set_shots first");
   const int n = c->p.n, nt = c->p.nt;
-  std::vector<float> t((size_t)nt * n);
-  for (int s = 0; s < c->nshot; ++s) {
-    const float* src = seis + (size_t)s * n * nt;
-    for (int i = 0; i < n; ++i)
-      for (int k = 0; k < nt; ++k) t[(size_t)k * n + i] = src[(size_t)i * nt + k];
-    HIPCHK(c, hipMemcpy(c->d_data + (size_t)s * nt * n, t.data(), t.size() * sizeof(float),
-                        hipMemcpyHostToDevice));
+  int rc = stage_buffer(c);
+  if (rc != RTM_OK) return rc;
+  for (int s = 0; s < c->nshot; ++s) {  // [n][nt] on the host -> [nt][n] on the device
+    HIPCHK(c, hipMemcpyAsync(c->d_stage, seis + (size_t)s * n * nt, sizeof(float) * n * nt,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_transpose(c->d_stage, c->d_data + (size_t)s * nt * n, n, nt, c->stream));
   }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   c->data_ok = true;
   return RTM_OK;
 }
@@ -1160,14 +1168,14 @@ int rtm_read_records(rtm_ctx* c, float* out) {
   if (!c || !out) return RTM_E_ARG;
   if (!c->p.record) return fail(c, RTM_E_STATE, "context created without records");
   const int n = c->p.n, nt = c->p.nt;
-  std::vector<float> t((size_t)nt * n);
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int s = 0; s < c->nshot; ++s) {
-    HIPCHK(c, hipMemcpy(t.data(), c->d_rec + (size_t)s * nt * n, t.size() * sizeof(float), hipMemcpyDeviceToHost));
-    float* o = out + (size_t)s * n * nt;
-    for (int k = 0; k < nt; ++k)
-      for (int i = 0; i < n; ++i) o[(size_t)i * nt + k] = t[(size_t)k * n + i];
+  int rc = stage_buffer(c);
+  if (rc != RTM_OK) return rc;
+  for (int s = 0; s < c->nshot; ++s) {  // [nt][n] on the device -> [n][nt] on the host
+    HIPCHK(c, launch_transpose(c->d_rec + (size_t)s * nt * n, c->d_stage, nt, n, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out + (size_t)s * n * nt, c->d_stage, sizeof(float) * n * nt,
+                             hipMemcpyDeviceToHost, c->stream));
   }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return RTM_OK;
 }
 

@@ -168,6 +168,8 @@ hipError_t launch_dot64(const float* a, const float* b, long long n, double* scr
                         hipStream_t s);
 // y = y + a*x over n elements
 hipError_t launch_axpy(float* y, const float* x, float a, long long n, hipStream_t s);
+// out[c][r] = in[r][c] (rows x cols, row-major)
+hipError_t launch_transpose(const float* in, float* out, int rows, int cols, hipStream_t s);
 hipError_t launch_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
                                float* img, float* illum, hipStream_t s);
 

@@ -965,6 +965,23 @@ __global__ void k_axpy(float* y, const float* x, float a, long long n) {
     y[i] = y[i] + a * x[i];
 }
 
+// out[c][r] = in[r][c] for an rows x cols matrix, through 32 x 32 LDS tiles so
+// that both the reads and the writes are coalesced.
+__global__ __launch_bounds__(256) void k_transpose(const float* in, float* out, int rows, int cols) {
+  __shared__ float tile[32][33];
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + j, c = c0 + tx;
+    if (r < rows && c < cols) tile[j][tx] = in[(long long)r * cols + c];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int c = c0 + j, r = r0 + tx;
+    if (r < rows && c < cols) out[(long long)c * rows + r] = tile[tx][j];
+  }
+}
+
 __global__ void k_stack_finish(int nI, const float* s1, const float* s2, int nrec, int inorm,
                                float* img, float* ill) {  // kernel.cu:1042-1059
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nI; e += gridDim.x * blockDim.x) {
@@ -1145,6 +1162,12 @@ hipError_t launch_dot64(const float* a, const float* b, long long n, double* scr
 
 hipError_t launch_axpy(float* y, const float* x, float a, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n)), dim3(256), 0, s, y, x, a, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(const float* in, float* out, int rows, int cols, hipStream_t s) {
+  hipLaunchKernelGGL(k_transpose, dim3((cols + 31) / 32, (rows + 31) / 32), dim3(256), 0, s, in, out,
+                     rows, cols);
   return hipGetLastError();
 }
 
