@@ -96,6 +96,20 @@ class Deblur:
         self._chk(_lib().rtm_cnn_get_params(self.h, p), "rtm_cnn_get_params")
         return p
 
+    def save(self, path):
+        """Parameters, widths and dilations to a .npz (plain arrays)."""
+        np.savez(path, params=self.params(), channels=np.asarray(self.channels, np.int32),
+                 dilation=np.asarray(self.dilation, np.int32))
+
+    @classmethod
+    def load(cls, path, max_pixels, max_rows):
+        """A network saved by save(), for batches up to max_pixels / max_rows."""
+        with np.load(path, allow_pickle=False) as f:
+            net = cls([int(v) for v in f["channels"]], max_pixels, max_rows,
+                      dilation=[int(v) for v in f["dilation"]])
+            net.set_params(f["params"])
+        return net
+
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float32)
         n, _, hh, ww = x.shape

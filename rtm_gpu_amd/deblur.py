@@ -181,7 +181,7 @@ def panel_residuals(ctx, panels, dm):
 
 def run_models(S, ntrain=96, neval=8, panels=8, epochs=100, patch=64, batch=256,
                patches_per_epoch=2048, lr=1e-3, channels=(3,) + (32,) * 5 + (1,),
-               dilation=None, cg_iters=10, target="true", seed=0):
+               dilation=None, cg_iters=10, target="true", seed=0, save_net=""):
     """Several random models: train on engine-made pairs (patches), evaluate on
     held-out models against CGLS with per-model statistics."""
     res = dict(grid=f"{S['mod_nz']}x{S['mod_nx']}", nt=int(S["d"].nt), shots=S["nshot"],
@@ -209,6 +209,8 @@ def run_models(S, ntrain=96, neval=8, panels=8, epochs=100, patch=64, batch=256,
             log(f"[deblur] epoch {ep + 1} of {epochs}: loss {hist[-1]:.3e}")
     res["train_s"] = time.time() - t
     res["loss_first"], res["loss_last"] = float(hist[0]), float(hist[-1])
+    if save_net:
+        net.save(save_net)
     del XF, M
     # held-out models: image, network, CGLS per panel
     ctx = context(S, panels)
@@ -262,8 +264,9 @@ def run_models(S, ntrain=96, neval=8, panels=8, epochs=100, patch=64, batch=256,
 
 def run(S=None, npairs=128, panels=32, epochs=600, batch=8, lr=1e-3,
         channels=(3,) + (16,) * 11 + (1,), dilation=(1, 2, 4, 8, 16, 32, 16, 8, 4, 2, 1, 1),
-        cg_iters=10, seed=0, target="cg"):
-    """Train on engine-made pairs and compare with CGLS; returns a results dict."""
+        cg_iters=10, seed=0, target="cg", save_net=""):
+    """Train on engine-made pairs and compare with CGLS; returns a results dict
+    and the images; save_net: write the trained network there (cnn.Deblur.save)."""
     S = S or build_survey()
     res = dict(grid=f"{S['mod_nz']}x{S['mod_nx']}", nt=int(S["d"].nt), shots=S["nshot"],
                npairs=npairs, epochs=epochs, channels=list(channels),
@@ -287,6 +290,8 @@ def run(S=None, npairs=128, panels=32, epochs=600, batch=8, lr=1e-3,
     res["train_s"] = time.time() - t
     res["loss_first"], res["loss_last"] = float(hist[0]), float(hist[-1])
     res["loss_identity"] = float(np.mean((XF[:, 0] - M) ** 2))
+    if save_net:
+        net.save(save_net)
     log(f"[deblur] pairs {res['pairs_s']:.1f}s, training {res['train_s']:.1f}s, loss "
         f"{res['loss_identity']:.3e} (scaled image) -> {res['loss_last']:.3e}")
 
@@ -350,6 +355,7 @@ def main():
                     help="many random models (1024 x 256, 64 shots, 25 Hz, 3000 steps; 6 x 32-"
                          "channel layers on 64 x 64 patches, batch 256), held-out evaluation")
     ap.add_argument("--ntrain", type=int, default=96)
+    ap.add_argument("--save-net", default="", help="write the trained network to this .npz")
     ap.add_argument("--neval", type=int, default=8)
     a = ap.parse_args()
     if a.models:
@@ -357,12 +363,12 @@ def main():
                          n2=30)
         res = run_models(S, ntrain=a.ntrain, neval=a.neval, epochs=a.epochs if a.epochs != 600
                          else 100, target=a.target if "--target" in __import__("sys").argv
-                         else "true")
+                         else "true", save_net=a.save_net)
         print(json.dumps(res))
         return
     S = build_survey(nshot=a.shots, nt=a.nt)
     res, im = run(S, npairs=a.npairs, panels=a.panels, epochs=a.epochs, batch=a.batch, lr=a.lr,
-                  cg_iters=a.cg_iters, target=a.target)
+                  cg_iters=a.cg_iters, target=a.target, save_net=a.save_net)
     if a.save:
         np.savez_compressed(a.save, **im)
     print(json.dumps(res))

@@ -144,6 +144,18 @@ def test_adam_step():
     assert np.allclose(p1, exp, rtol=1e-5, atol=1e-7)
 
 
+def test_save_and_load(tmp_path):
+    from rtm_gpu_amd import cnn
+    net, p = _net([3, 32, 8, 1], seed=4, dil=[2, 1, 3])
+    x, _ = _data(2, 3, 18, 33, seed=2)
+    y = net.forward(x)
+    net.save(tmp_path / "net.npz")
+    back = cnn.Deblur.load(tmp_path / "net.npz", 4096, 64)
+    assert back.channels == [3, 32, 8, 1] and back.dilation == [2, 1, 3]
+    assert np.array_equal(back.params(), p)
+    assert np.array_equal(back.forward(x), y)
+
+
 def test_zero_network_is_the_identity_on_channel_zero():
     from rtm_gpu_amd import cnn
     net = cnn.Deblur([2, 16, 1], 4096, 64)
