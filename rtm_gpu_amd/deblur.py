@@ -159,6 +159,33 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
+class _Iterates:
+    """CGLS callback keeping each iterate and the time it was reached."""
+
+    def __init__(self, note=None):
+        self.m, self.t, self.t0, self.note = [], [], time.time(), note
+
+    def __call__(self, it, m, h):
+        self.m.append(np.array(m, copy=True))
+        self.t.append(time.time() - self.t0)
+        if self.note:
+            log(f"{self.note} {it}")
+
+    def first_within(self, final, tol=0.05, axis_models=False):
+        """Iteration (1-based) and time at which the iterate first comes within
+        tol relative L2 of `final` (per model when axis_models)."""
+        if not axis_models:
+            for i, m in enumerate(self.m):
+                if rel(m, final) <= tol:
+                    return i + 1, self.t[i]
+            return None, None
+        out = []
+        for g in range(final.shape[0]):
+            hit = next((i for i, m in enumerate(self.m) if rel(m[g], final[g]) <= tol), None)
+            out.append((None, None) if hit is None else (hit + 1, self.t[hit]))
+        return out
+
+
 def patches(XF, M, size, count, rng):
     """count random size x size crops (all channels) of the pairs."""
     n, c, hh, ww = XF.shape
@@ -234,15 +261,18 @@ def run_models(S, ntrain=96, neval=8, panels=8, epochs=100, patch=64, batch=256,
         m_net = np.stack([net.forward(xe[i:i + 1])[0] for i in range(panels)])
         t_net = time.time() - t
         t = time.time()
+        iters = _Iterates("[deblur] held-out CGLS iteration")
         m_cg, _ = lsrtm.cgls(lsrtm.BornOperator(ctx, groups=panels), None, cg_iters,
-                             callback=lambda it, *_: log(f"[deblur] held-out CGLS iteration {it}"))
+                             callback=iters)
         t_cg = time.time() - t
+        reach = iters.first_within(m_cg, axis_models=True)
         m_rtm = np.float32(c) * img
         dn = panel_residuals(ctx, panels, mt)
         rn = {name: panel_residuals(ctx, panels, est - mt) / np.where(dn > 0, dn, 1.0)
               for name, est in (("rtm", m_rtm), ("net", m_net), ("cg", m_cg))}
         for i in range(k):
-            per.append(dict(rel_net_vs_cg=rel(m_net[i], m_cg[i]), rel_rtm_vs_cg=rel(m_rtm[i], m_cg[i]),
+            per.append(dict(cg_iters_to_5pct=reach[i][0],
+                            rel_net_vs_cg=rel(m_net[i], m_cg[i]), rel_rtm_vs_cg=rel(m_rtm[i], m_cg[i]),
                             rel_net_vs_true=rel(m_net[i], mt[i]), rel_cg_vs_true=rel(m_cg[i], mt[i]),
                             resid_rtm=float(rn["rtm"][i]), resid_net=float(rn["net"][i]),
                             resid_cg=float(rn["cg"][i])))
@@ -252,7 +282,9 @@ def run_models(S, ntrain=96, neval=8, panels=8, epochs=100, patch=64, batch=256,
         log(f"[deblur] held-out models {b}..{b + k - 1}: rel to CGLS net "
             f"{np.mean([q['rel_net_vs_cg'] for q in per[-k:]]):.3f}")
     for key in per[0]:
-        res[key] = float(np.mean([q[key] for q in per]))
+        if key != "cg_iters_to_5pct":
+            res[key] = float(np.mean([q[key] for q in per]))
+    res["cg_iters_to_5pct"] = [q["cg_iters_to_5pct"] for q in per]
     res["speedup_vs_cg"] = res["t_cg_s"] / (res["t_image_s"] + res["t_net_s"])
     res["per_model"] = per
     log(f"[deblur] {neval} held-out models: rel L2 to CGLS net {res['rel_net_vs_cg']:.3f}, "
@@ -313,8 +345,10 @@ def run(S=None, npairs=128, panels=32, epochs=600, batch=8, lr=1e-3,
     t_net = time.time() - t
     m_rtm = np.float32(c) * img
     t = time.time()
-    m_cg, ch = lsrtm.cgls(op, dobs, cg_iters)
+    iters = _Iterates()
+    m_cg, ch = lsrtm.cgls(op, dobs, cg_iters, callback=iters)
     t_cg = time.time() - t
+    k5, t5 = iters.first_within(m_cg)
 
     def resid(m):
         ctx.set_reflectivity(m)
@@ -328,7 +362,9 @@ def run(S=None, npairs=128, panels=32, epochs=600, batch=8, lr=1e-3,
         rel_net_vs_true=rel(m_net, m_true), rel_rtm_vs_true=rel(m_rtm, m_true),
         rel_cg_vs_true=rel(m_cg, m_true),
         resid_rtm=resid(m_rtm), resid_net=resid(m_net), resid_cg=resid(m_cg),
-        cg_rr=[h["rr"] / ch[0]["rr"] for h in ch])
+        cg_rr=[h["rr"] / ch[0]["rr"] for h in ch],
+        cg_iters_to_5pct=k5, cg_time_to_5pct_s=t5,
+        net_within_5pct=bool(rel(m_net, m_cg) <= 0.05))
     log(f"[deblur] image {t_img:.2f}s + net {t_net:.3f}s vs CGLS({cg_iters}) {t_cg:.2f}s; "
         f"rel L2 to CGLS: net {res['rel_net_vs_cg']:.3f}, scaled RTM {res['rel_rtm_vs_cg']:.3f}; "
         f"data residual: RTM {res['resid_rtm']:.3f}, net {res['resid_net']:.3f}, "
