@@ -227,6 +227,14 @@ def algorithmic_bytes(kind, d, n2, nfd, nshot, n):
         return nshot * (nI * 48 + n * 4) + nI * 4
     if kind == hip.K_RCV_FRAME:
         return nshot * (nF * 12 + n * 4) + nF * 4
+    # Born: dU1, dU0, u(k), u(k-1), u(k-2) read, dU2 written (24 B) + v, m shared
+    if kind == hip.K_BORN_INTERIOR:
+        return nshot * nI * 24 + nI * 8
+    # adjoint: mu1, mu0, u(k), u(k+1), u(k+2), G read, mu2, G written (36 B) + v shared
+    if kind == hip.K_ADJ_INTERIOR:
+        return nshot * (nI * 36 + n * 4) + nI * 4
+    if kind in (hip.K_BORN_FRAME, hip.K_ADJ_FRAME):
+        return nshot * (nF * 12 + n * 4) + nF * 4
     return 0
 
 
@@ -288,6 +296,77 @@ def copy_probe(mib):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
+
+
+def run_lsrtm(a):
+    """CGLS least-squares migration on the reverse-VSP batch (351 x 1701,
+    128 shots, 4000 steps): Born data of the faulted model's reflectivity,
+    then timed CGLS iterations (one Born and one adjoint pass each)."""
+    from rtm_gpu_amd import deblur, hip, lsrtm, synth
+    P = build_problem(a.shots, nt=a.nt)
+    d, cfg = P["d"], P["cfg"]
+    ctx = hip.Context(nz=d.nz, nx=d.nx, n2=cfg.n2, nfdmax=cfg.nfdmax, nt=d.nt, nt2=d.nt2,
+                      ilste=0, icompen=1, s_l=d.s_l, s_r=d.s_r, s_z=d.s_z, ds=1, n=cfg.n,
+                      tao2=d.tao2, h2=d.h2, taoh=d.taoh, taoh2=d.taoh2, hzx2_1=d.hzx2_1,
+                      max_shots=a.shots, record=1, zseg=a.zseg)
+    mm = P["models"]["mig"]
+    ctx.set_wavelet(P["wav"])
+    ctx.set_shots(P["r_u"], P["r_x"])
+    ctx.set_model(mm["v"], mm["r1"], P["w"], mm["c"], mm["index"], mm["nvel"], mm["vmin"], 1.0)
+    raw = synth.faulted_rvsp(seed=2, mod_nz=351, mod_nx=1701, h=12.5)
+    m_true = deblur.reflectivity(raw, synth.smooth(raw, 4.0))
+    zseg = ctx.autotune(8) if a.zseg <= 0 else a.zseg
+    log(f"[lsrtm] autotune: {zseg} rows per wave")
+    ctx.set_reflectivity(m_true)
+    ctx.born_begin()
+    ctx.born_advance(d.nt)
+    ctx.gathers_from_records()  # observed data: Born data of the true reflectivity
+    ctx.sync()
+    log("[lsrtm] Born data modelled")
+    op = lsrtm.BornOperator(ctx)
+    rr0 = ctx.data_dot(hip.DATA_GATHERS, hip.DATA_GATHERS)
+    marks = []
+
+    def cb(it, m, h):
+        ctx.sync()
+        marks.append(time.perf_counter())
+        log(f"[lsrtm] iteration {it}: ||r||^2/||d||^2 = {h['rr'] / rr0:.4f}")
+    # Each iteration is one Born and one adjoint pass; the timed span runs from
+    # the end of the first iteration to the end of the last (kernel timing on).
+    niter = max(2, a.steps)
+    ctx.set_timing(-1)
+    _, hist = lsrtm.cgls(op, None, niter, callback=cb)
+    el = (marks[-1] - marks[0]) / (len(marks) - 1)
+    a.steps = len(marks) - 1
+    nI = (d.nz - 2 * cfg.n2) * (d.nx - 2 * cfg.n2)
+    upd_iter = (d.nt - 2) * (2 * d.nz * d.nx + nI + d.nz * d.nx) * a.shots
+    value = upd_iter / el / 1e9
+    kinds = {}
+    for k, name in hip.KERNEL_NAMES.items():
+        ms, n = ctx.kernel_time(k)
+        if n:
+            b = algorithmic_bytes(k, d, cfg.n2, cfg.nfdmax, a.shots, cfg.n)
+            kinds[name] = dict(avg_ms=round(ms / n, 5), launches=n,
+                               GBps=round(b / (ms / n * 1e-3) / 1e9, 1) if b else None)
+    ad = kinds.get("k_interior<EPI_ADJ>", {})
+    out = {"metric": "least-squares RTM (CGLS): Gpt-updates/s", "value": round(value, 3),
+           "unit": "Gpt-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": 1,
+           "ms_per_step": round(el * 1e3, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 (fp64 CG products)",
+           "data": "synthetic (Born data of the seeded faulted model's reflectivity)",
+           "config": {"workload": f"rvsp_faulted_351x1701_nt{d.nt}_{a.shots}shots_cgls",
+                      "shots": a.shots, "rows_per_wave": zseg,
+                      "passes_per_iteration": "Born (background + scattered) + adjoint "
+                                              "(reconstruction + adjoint field)"},
+           "s_per_iteration": round(el, 3),
+           "misfit": [round(h["rr"] / rr0, 5) for h in hist],
+           "roofline": {"bound": "hbm", "kernel": "k_interior<EPI_ADJ>",
+                        "achieved": ad.get("GBps"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(ad["GBps"] / HBM_PEAK_GBS, 4) if ad.get("GBps") else None,
+                        "traffic": None, "avg_launch_ms": ad.get("avg_ms")},
+           "kernels": kinds}
+    ctx.close()
+    print(json.dumps(out), flush=True)
 
 
 def run_deblur():
@@ -378,11 +457,13 @@ def main():
     ap.add_argument("--host-gathers", action="store_true",
                     help="every timed step uploads the gathers from host memory (the PCIe-"
                          "inclusive rate of a caller that hands over host buffers)")
-    ap.add_argument("--workload", default="rvsp", choices=["rvsp", "apertures", "deblur"],
+    ap.add_argument("--workload", default="rvsp",
+                    choices=["rvsp", "apertures", "deblur", "lsrtm"],
                     help="rvsp: the headline reverse-VSP run; apertures: uneven-aperture "
                          "surface shots in ragged batches vs one shot per launch; deblur: the "
                          "deblurring network against 10-iteration CGLS least-squares migration "
-                         "(training pairs, training and evaluation, about 4 minutes)")
+                         "(training pairs, training and evaluation, about 4 minutes); lsrtm: "
+                         "CGLS iterations on the reverse-VSP batch (--steps iterations)")
     ap.add_argument("--batch", type=int, default=128, help="apertures: shots per launch")
     ap.add_argument("--sample", type=int, default=8,
                     help="apertures: shots timed one per launch")
@@ -397,6 +478,9 @@ def main():
         sys.exit(2)
     if a.workload == "deblur":
         run_deblur()
+        return
+    if a.workload == "lsrtm":
+        run_lsrtm(a)
         return
     if a.workload == "apertures":
         if a.shots == 128 and "--shots" not in sys.argv:
