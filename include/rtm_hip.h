@@ -54,7 +54,10 @@ enum {
 enum {
   RTM_K_FWD_INTERIOR = 0, RTM_K_FWD_FRAME = 1, RTM_K_SRC_INTERIOR = 2, RTM_K_STRIP_RESTORE = 3,
   RTM_K_RCV_INTERIOR = 4, RTM_K_RCV_FRAME = 5, RTM_K_BORN_INTERIOR = 6, RTM_K_BORN_FRAME = 7,
-  RTM_K_ADJ_INTERIOR = 8, RTM_K_ADJ_FRAME = 9, RTM_K_NKINDS = 10
+  RTM_K_ADJ_INTERIOR = 8, RTM_K_ADJ_FRAME = 9,
+  /* the migration's backward step outside checkpoint steps: the reconstruction
+   * with the source-side image terms, the receiver step with the rest */
+  RTM_K_SRCI_INTERIOR = 10, RTM_K_RCVI_INTERIOR = 11, RTM_K_NKINDS = 12
 };
 
 typedef struct rtm_params {

@@ -81,7 +81,7 @@ struct rtm_ctx {
   double* d_dot = nullptr;      // FP64 dot-product scratch
   bool model_ok = false, wav_ok = false, shots_ok = false, data_ok = false, save_strips = true;
   int zs = 32, nxt = 1;
-  int zs_epi[8] = {0};  // rows per wave per interior kernel kind (0: zs)
+  int zs_epi[16] = {0};  // rows per wave per interior kernel kind (0: zs)
   // captured migration (hipGraph)
   bool graph_on = false;
   hipGraph_t graph = nullptr;
@@ -219,7 +219,7 @@ void fill_int_args(rtm_ctx* c, IntArgs& a) {
 }
 // Rows per wave of one interior kernel kind (autotuned per kind, else zs).
 int zs_for(const rtm_ctx* c, int epi) {
-  return (epi >= 0 && epi < 8 && c->zs_epi[epi] > 0) ? c->zs_epi[epi] : c->zs;
+  return (epi >= 0 && epi < 16 && c->zs_epi[epi] > 0) ? c->zs_epi[epi] : c->zs;
 }
 
 void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
@@ -270,14 +270,22 @@ int forward_step(rtm_ctx* c, int k) {
 
 int backward_step(rtm_ctx* c, int k) {
   const float wv = (k < c->p.nt2) ? c->wav[k + 1] : 0.0f;  // kernel.cu:889-890
+  // Rel_* (kernel.cu:489-517) is split between the two interior kernels: the
+  // terms of u(k) alone (sum u += u, I2 += u*u) go with its reconstruction,
+  // the rest with the receiver step.  At a checkpoint step the snapshot
+  // replaces u(k) after the reconstruction, and the non-compensated image
+  // (I1 += q*u) needs u(k) in the receiver step anyway: there the receiver
+  // step does all.
+  const bool split = c->snap_slot[k] < 0 && c->p.icompen == 1;
+  const int esrc = split ? EPI_SRCI : EPI_SRC, ercv = split ? EPI_RCVI : EPI_RCV;
   {  // BKAdd_EFF (kernel.cu:896): u(k) on the interior
     IntArgs a;
     fill_int_args(c, a);
     a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
     a.wav = wv; a.k = k;
-    Timed t(c, RTM_K_SRC_INTERIOR);
-    a.zs = zs_for(c, EPI_SRC);
-    HIPCHK(c, launch_interior(EPI_SRC, c->nfd, a, c->nshot, c->stream));
+    Timed t(c, split ? RTM_K_SRCI_INTERIOR : RTM_K_SRC_INTERIOR);
+    a.zs = zs_for(c, esrc);
+    HIPCHK(c, launch_interior(esrc, c->nfd, a, c->nshot, c->stream));
   }
   {  // strips of slot k into the rings of u(k): BKEqual of step k-1 (kernel.cu:892)
     Timed t(c, RTM_K_STRIP_RESTORE);
@@ -292,9 +300,9 @@ int backward_step(rtm_ctx* c, int k) {
     a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
     a.uk = c->F[c->f1];
     a.wav = 0.f; a.k = k;
-    Timed t(c, RTM_K_RCV_INTERIOR);
-    a.zs = zs_for(c, EPI_RCV);
-    HIPCHK(c, launch_interior(EPI_RCV, c->nfd, a, c->nshot, c->stream));
+    Timed t(c, split ? RTM_K_RCVI_INTERIOR : RTM_K_RCV_INTERIOR);
+    a.zs = zs_for(c, ercv);
+    HIPCHK(c, launch_interior(ercv, c->nfd, a, c->nshot, c->stream));
   }
   {  // BKAdd on the frame + BKHybrid1/2 (kernel.cu:913-920)
     FrameArgs a;
@@ -1089,15 +1097,15 @@ int rtm_autotune(rtm_ctx* c, int nsteps, int* zseg) {
   // adjoint share the receiver step's register-queue shape and take its
   // choice.
   const int fwd = c->p.ilste ? EPI_FWD64 : EPI_FWD;
-  const int kinds[3] = {fwd, EPI_SRC, EPI_RCV};
-  for (int ki = 0; ki < 3 && rc == RTM_OK; ++ki) {
+  const int kinds[5] = {fwd, EPI_SRCI, EPI_RCVI, EPI_SRC, EPI_RCV};
+  for (int ki = 0; ki < 5 && rc == RTM_OK; ++ki) {
     const int epi = kinds[ki];
     int best = c->zs;
     float best_ms = 3.0e38f;
     for (int z : kCand) {
       IntArgs a;
       fill_int_args(c, a);
-      const bool rcv = epi == EPI_RCV;
+      const bool rcv = epi == EPI_RCV || epi == EPI_RCVI;
       a.u1 = rcv ? c->Q[1] : c->F[1]; a.u0 = rcv ? c->Q[0] : c->F[0];
       a.u2 = rcv ? c->Q[2] : c->F[2];
       a.uk = c->F[1];
@@ -1123,10 +1131,10 @@ int rtm_autotune(rtm_ctx* c, int nsteps, int* zseg) {
   c->kf = -1;
   c->kb = -2;
   drop_graph(c);
-  if (zseg) {
+  if (zseg) {  // the kernels of an ordinary migration step
     zseg[0] = c->zs_epi[fwd];
-    zseg[1] = c->zs_epi[EPI_SRC];
-    zseg[2] = c->zs_epi[EPI_RCV];
+    zseg[1] = c->zs_epi[EPI_SRCI];
+    zseg[2] = c->zs_epi[EPI_RCVI];
   }
   return rc;
 }

@@ -67,10 +67,12 @@ struct Snaps {
 };
 
 // Interior epilogues: forward (FP32 / FP64 combine), source reconstruction,
-// receiver step + imaging, Born scattered field (FP32 / FP64 combine), and
-// the adjoint of Born modelling.
+// receiver step + imaging, Born scattered field (FP32 / FP64 combine), the
+// adjoint of Born modelling, and the split imaging of the migration: the
+// reconstruction with the source-side image terms (sum u, I2 += u*u) and the
+// receiver step with the rest (sum q, I1; compensated imaging only).
 enum { EPI_FWD = 0, EPI_SRC = 1, EPI_RCV = 2, EPI_FWD64 = 3, EPI_BORN = 4, EPI_BORN64 = 5,
-       EPI_ADJ = 6 };
+       EPI_ADJ = 6, EPI_SRCI = 7, EPI_RCVI = 8 };
 enum { FR_FWD = 0, FR_RCV = 1, FR_BORN = 2, FR_ADJ = 3 };
 
 struct IntArgs {

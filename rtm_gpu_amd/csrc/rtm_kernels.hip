@@ -239,8 +239,11 @@ __device__ __forceinline__ float strip_get(const Strips& st, int shot, int k, lo
 // end of the last segment are computed from clamped rows and not stored.
 template <int EPI>
 struct EpiTraits {
-  static constexpr bool f64 = EPI == EPI_FWD64 || EPI == EPI_SRC || EPI == EPI_BORN64;  // "2.0*"
-  static constexpr bool rcv = EPI == EPI_RCV;
+  static constexpr bool f64 =
+      EPI == EPI_FWD64 || EPI == EPI_SRC || EPI == EPI_SRCI || EPI == EPI_BORN64;  // "2.0*"
+  static constexpr bool rcv = EPI == EPI_RCV || EPI == EPI_RCVI;  // receiver step + imaging
+  static constexpr bool rcvi = EPI == EPI_RCVI;  // ... without the source-side terms
+  static constexpr bool srci = EPI == EPI_SRCI;  // reconstruction + source-side terms
   static constexpr bool born = EPI == EPI_BORN || EPI == EPI_BORN64;
   static constexpr bool adj = EPI == EPI_ADJ;
   static constexpr bool src = !rcv && !born && !adj;  // injects the source wavelet
@@ -331,7 +334,7 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   }
   Rsrc RU, RI1, RI2, RSU, RSQ;
   unsigned vcs = kDrop, vss = kDrop;
-  if (T::rcv) {
+  if (T::rcv || T::srci) {
     RU = rsrc(a.uk + so, fb);
     RI1 = rsrc(a.i1 + so, fb); RI2 = rsrc(a.i2 + so, fb);
     RSU = rsrc(a.su + so, fb); RSQ = rsrc(a.sq + so, fb);
@@ -359,7 +362,7 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
     u0b[d] = bld4(R0, vcol, rowb(zs + d));
     hvb[d] = bld4(R1, vhal, rowb(zs + d));
   }
-  float4 ukb[1], i1b[1], i2b[1], sub[1], sqb[1];
+  float4 ukb[1] = {}, i1b[1] = {}, i2b[1] = {}, sub[1] = {}, sqb[1] = {};
   float4 xab[2], xbb[2], xcb[2], xxb[2];
   if (T::born || T::adj) {
     xab[0] = bld4(RA, vxl, rowb(zs));
@@ -384,11 +387,16 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
       // row of look-ahead for them would cost 20 VGPRs the kernel needs).
       if (T::rcv) {
         const unsigned rzz = rowb(zz);
-        ukb[0] = bld4(RU, vcol, rzz);
+        if (!T::rcvi) ukb[0] = bld4(RU, vcol, rzz);  // the split step never needs u(k)
         i1b[0] = bld4(RI1, vcol, rzz);
-        i2b[0] = bld4(RI2, vcol, rzz);
+        if (!T::rcvi) i2b[0] = bld4(RI2, vcol, rzz);
         sub[0] = bld4(RSU, vcs, rzz);
         sqb[0] = bld4(RSQ, vcs, rzz);
+      }
+      if (T::srci) {  // this row's sum u and I2, for the source-side image terms
+        const unsigned rzz = rowb(zz);
+        i2b[0] = bld4(RI2, vcol, rzz);
+        sub[0] = bld4(RSU, vcs, rzz);
       }
       // (b) operands of later rows
       q[(j + 2 * NFD + PD) % QL] = bld4(R1, vcol, rowb(zz + NFD + PD));
@@ -447,18 +455,34 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
         float r1o[4], r2o[4], rsu[4], rsq[4];
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj) {
-          const float u = compr(ukv, jj);
-          rsu[jj] = a.icompen ? compr(su, jj) + u : u;
+          const float u = T::rcvi ? 0.0f : compr(ukv, jj);
+          // split form (compensated imaging only): sum u already holds this
+          // step's u(k), which the reconstruction added
+          rsu[jj] = T::rcvi ? compr(su, jj) : (a.icompen ? compr(su, jj) + u : u);
           rsq[jj] = a.icompen ? compr(sq, jj) + out[jj] : out[jj];
           r1o[jj] = colok[jj] ? __builtin_fmaf(rsq[jj], rsu[jj], compr(i1, jj)) : compr(i1, jj);
-          r2o[jj] = colok[jj] ? __builtin_fmaf(u, u, compr(i2, jj)) : compr(i2, jj);
+          if (!T::rcvi) r2o[jj] = colok[jj] ? __builtin_fmaf(u, u, compr(i2, jj)) : compr(i2, jj);
           rsu[jj] = colok[jj] ? rsu[jj] : compr(su, jj);
           rsq[jj] = colok[jj] ? rsq[jj] : compr(sq, jj);
         }
         bst4(RI1, vst | rowdrop, rz, r1o[0], r1o[1], r1o[2], r1o[3]);
+        if (!T::rcvi) {
+          bst4(RI2, vst | rowdrop, rz, r2o[0], r2o[1], r2o[2], r2o[3]);
+          bst4(RSU, vss | rowdrop, rz, rsu[0], rsu[1], rsu[2], rsu[3]);
+        }
+        bst4(RSQ, vss | rowdrop, rz, rsq[0], rsq[1], rsq[2], rsq[3]);
+      }
+      if (T::srci) {  // source-side image terms of step k on the u(k) just computed
+        const float4 i2 = i2b[0], su = sub[0];
+        float r2o[4], rsu[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const float u = out[jj];
+          rsu[jj] = colok[jj] ? compr(su, jj) + u : compr(su, jj);
+          r2o[jj] = colok[jj] ? __builtin_fmaf(u, u, compr(i2, jj)) : compr(i2, jj);
+        }
         bst4(RI2, vst | rowdrop, rz, r2o[0], r2o[1], r2o[2], r2o[3]);
         bst4(RSU, vss | rowdrop, rz, rsu[0], rsu[1], rsu[2], rsu[3]);
-        bst4(RSQ, vss | rowdrop, rz, rsq[0], rsq[1], rsq[2], rsq[3]);
       }
       if (T::adj) {  // G += D2u(k+1) * mu(k+2); mu(k+2) is this update's u0
         float go[4];
@@ -1000,7 +1024,8 @@ __global__ void k_stack_finish(int nI, const float* s1, const float* s2, int nre
 #define RTM_NFD_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
 int interior_queue_rows(int epi, int nfd) {
-  const bool pd1 = epi == EPI_RCV || epi == EPI_BORN || epi == EPI_BORN64 || epi == EPI_ADJ;
+  const bool pd1 = epi == EPI_RCV || epi == EPI_RCVI || epi == EPI_BORN || epi == EPI_BORN64 ||
+                   epi == EPI_ADJ;
   return 2 * nfd + 1 + (pd1 ? 1 : 2);  // EpiTraits<EPI>::PD
 }
 
@@ -1020,6 +1045,8 @@ hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s
     else if (epi == EPI_BORN) hipLaunchKernelGGL((k_interior<N, EPI_BORN>), grid, block, 0, s, a); \
     else if (epi == EPI_BORN64) hipLaunchKernelGGL((k_interior<N, EPI_BORN64>), grid, block, 0, s, a); \
     else if (epi == EPI_ADJ) hipLaunchKernelGGL((k_interior<N, EPI_ADJ>), grid, block, 0, s, a); \
+    else if (epi == EPI_SRCI) hipLaunchKernelGGL((k_interior<N, EPI_SRCI>), grid, block, 0, s, a); \
+    else if (epi == EPI_RCVI) hipLaunchKernelGGL((k_interior<N, EPI_RCVI>), grid, block, 0, s, a); \
     else hipLaunchKernelGGL((k_interior<N, EPI_RCV>), grid, block, 0, s, a);               \
     break;
   switch (nfd) {

@@ -15,13 +15,15 @@ RTM_STRIP_F32, RTM_STRIP_F16, RTM_STRIP_BF16 = 0, 1, 2
 FIELD_U, FIELD_U_PREV, FIELD_Q, FIELD_I1, FIELD_I2, FIELD_SU, FIELD_SQ = range(7)
 DATA_GATHERS, DATA_RECORDS = 0, 1
 (K_FWD_INTERIOR, K_FWD_FRAME, K_SRC_INTERIOR, K_STRIP_RESTORE, K_RCV_INTERIOR, K_RCV_FRAME,
- K_BORN_INTERIOR, K_BORN_FRAME, K_ADJ_INTERIOR, K_ADJ_FRAME) = range(10)
+ K_BORN_INTERIOR, K_BORN_FRAME, K_ADJ_INTERIOR, K_ADJ_FRAME, K_SRCI_INTERIOR,
+ K_RCVI_INTERIOR) = range(12)
 KERNEL_NAMES = {
     K_FWD_INTERIOR: "k_interior<EPI_FWD>", K_FWD_FRAME: "k_frame<FR_FWD>",
     K_SRC_INTERIOR: "k_interior<EPI_SRC>", K_STRIP_RESTORE: "k_strip_restore",
     K_RCV_INTERIOR: "k_interior<EPI_RCV>", K_RCV_FRAME: "k_frame<FR_RCV>",
     K_BORN_INTERIOR: "k_interior<EPI_BORN>", K_BORN_FRAME: "k_frame<FR_BORN>",
     K_ADJ_INTERIOR: "k_interior<EPI_ADJ>", K_ADJ_FRAME: "k_frame<FR_ADJ>",
+    K_SRCI_INTERIOR: "k_interior<EPI_SRCI>", K_RCVI_INTERIOR: "k_interior<EPI_RCVI>",
 }
 
 
