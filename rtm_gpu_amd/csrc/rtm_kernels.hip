@@ -266,19 +266,31 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   // wave index: uniform in the wave, so row offsets derived from it stay scalar
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // The four waves of block b march the same rows and columns (tile
-  // b / nsg) for four shots (group b % nsg), so their reads of the shared
+  // b / nsg) for four shots (group b % nsg; launches of at least four
+  // shots), so their reads of the shared
   // model rows (p, coefficient offsets, coefficient rows) coincide in time and
   // hit the CU's L1.  Workgroups reach the eight XCDs round-robin by b, so
   // the blocks one XCD holds at a time are more shots of the same few tiles,
   // whose model reads hit that XCD's L2.
-  const unsigned nsg = ((unsigned)a.nshot + 3u) >> 2;
-  const int shot = (int)(blockIdx.x % nsg) * 4 + wv;
-  const int tile = (int)(blockIdx.x / nsg);
-  if (shot >= a.nshot) return;
-  const int xt = tile % a.nxt, zg = tile / a.nxt;
+  // With fewer than four shots the waves of a block take four consecutive
+  // row segments of one shot instead (block b: shot b % B, tile b / B).
+  int shot, zseg;
+  int tile;
+  if (a.nshot >= 4) {
+    const unsigned nsg = ((unsigned)a.nshot + 3u) >> 2;
+    shot = (int)(blockIdx.x % nsg) * 4 + wv;
+    tile = (int)(blockIdx.x / nsg);
+    if (shot >= a.nshot) return;
+    zseg = tile / a.nxt;
+  } else {
+    shot = (int)(blockIdx.x % (unsigned)a.nshot);
+    tile = (int)(blockIdx.x / (unsigned)a.nshot);
+    zseg = (tile / a.nxt) * 4 + wv;
+  }
+  const int xt = tile % a.nxt;
   const ShotGeo sg = a.geo[shot];
   const int zhi = g.nz - g.n2;
-  const int zs = g.n2 + zg * a.zs;  // a.zs is a multiple of QL
+  const int zs = g.n2 + zseg * a.zs;  // a.zs is a multiple of QL
   const int cend = sg.nx - g.n2;                // first column past the shot's interior
   const int cb = g.n2 + xt * 256;
   if (zs >= zhi || cb >= cend) return;          // tiles past a narrower shot's width
@@ -1036,7 +1048,10 @@ hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s
   a.zs = mult * ql;
   const int nseg = (a.g.mnz + a.zs - 1) / a.zs;
   a.nshot = nshot;
-  dim3 grid(a.nxt * nseg * ((nshot + 3) / 4)), block(256);
+  // four shots per block on shared rows, or four row segments of one shot
+  // per block when the launch has fewer than four shots (k_interior)
+  dim3 grid(nshot >= 4 ? a.nxt * nseg * ((nshot + 3) / 4) : a.nxt * ((nseg + 3) / 4) * nshot),
+      block(256);
 #define CASE(N)                                                                             \
   case N:                                                                                   \
     if (epi == EPI_FWD) hipLaunchKernelGGL((k_interior<N, EPI_FWD>), grid, block, 0, s, a);     \
