@@ -426,9 +426,9 @@ struct Engine {
     for (int x = b; x < e; ++x)
       for (int y = 0; y < E.NX; ++y) {
         size_t p = (size_t)x * E.NX + y;
-        int judge = (y - g.s_l) % g.ds;
+        int rem = (y - g.s_l) % g.ds;  // receiver spacing test (kernel.cu:349)
         int s = (y - g.s_l) / g.ds;
-        if (x == g.s_z && y >= g.s_l && y <= g.s_r && judge == 0 && E.dseis[s] != R(0)) {
+        if (x == g.s_z && y >= g.s_l && y <= g.s_r && rem == 0 && E.dseis[s] != R(0)) {
           Q2[p] = E.dseis[s];  // replacement (kernel.cu:351-354)
         } else {
           R w1 = E.w1_at(Q1, x, y);
