@@ -31,7 +31,7 @@ def layered_model(mod_nz, mod_nx, nlayer, vlo, vhi, seed, dip=0.0):
 def build_case(name="small", mod_nz=40, mod_nx=60, n2=10, nfdmax=4, nfdmin=2, nt=240,
                h=10.0, hz=10.0, tao=0.001, f0=25.0, ilste=0, icompen=1, seed=1, src_x1=30,
                src_depth=30.0, nlayer=4, vlo=1500.0, vhi=2800.0, ds=1, n=None, s_l1=1,
-               s_z1=1, dip=0.15, nshot=1, depth_step=20.0, kind="layered"):
+               s_z1=1, dip=0.15, nshot=1, depth_step=20.0, kind="layered", smooth=0.0):
     fmax = 2 * f0 + 1
     if n is None:
         n = (mod_nx - s_l1) // ds + 1
@@ -39,6 +39,8 @@ def build_case(name="small", mod_nz=40, mod_nx=60, n2=10, nfdmax=4, nfdmin=2, nt
     if kind == "faulted":  # the seeded reverse-VSP model (8 layers, 3 faults, a body)
         from rtm_gpu_amd import synth
         raw = synth.faulted_rvsp(seed=seed, mod_nz=mod_nz, mod_nx=mod_nx, h=h)
+        if smooth:  # Gaussian-smoothed migration velocity: thousands of velocity bins
+            raw = synth.smooth(raw, smooth)
     else:
         raw = layered_model(mod_nz, mod_nx, nlayer, vlo, vhi, seed, dip)
     v, r1 = ob.pad_model(raw, mod_nz, mod_nx, n2, 0, tao, h)

@@ -1,0 +1,149 @@
+"""The reverse-VSP workload at its full size: 351 x 1701 interior
+(411 x 1761 with the 30-cell frame), nfdmax 10, adaptive least-squares
+coefficients of the smoothed faulted model (about 3000 velocity bins),
+receivers on every surface cell.
+
+- Over a truncated number of steps the oracle still finishes in seconds on
+  the full grid: four sources (one inside the absorbing frame, row N2-1, as
+  the first source of the reference's depth list) in one batch, so the
+  kernels run their four-shots-per-block mapping; every field equals the FP32
+  replica bit for bit.
+- Over the full 4000 steps, properties that hold at any size: identical
+  shots in different blocks give identical images, a shot in a batch of
+  eight equals its single-shot run (the other block mapping), the source
+  illumination does not depend on the data, and with uncompensated imaging
+  (a receiver field linear in the data) negated gathers give exactly the
+  negated image away from the source cell."""
+import numpy as np
+import pytest
+
+from cases import build_case, hip_context, oracle_engine, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FULL = dict(kind="faulted", smooth=4.0, seed=2, mod_nz=351, mod_nx=1701, n2=30, nfdmax=10,
+            h=12.5, hz=12.5, f0=20.0, src_x1=851)
+
+
+def _gathers(case, nshot, seed):
+    """Seeded data for nshot sources, [shot][n][NT]; about a third of the
+    samples are exact zeros, which the receiver step does not inject."""
+    cfg = case["cfg"]
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((nshot, cfg["n"], cfg["nt"])).astype(np.float32)
+    g[rng.random(g.shape) < 0.35] = 0.0
+    return g
+
+
+def test_full_grid_parity_truncated():
+    from rtm_gpu_amd import hip
+    case = build_case(name="rvsp_full_300", nt=300, nshot=4, src_depth=6.25, depth_step=793.75,
+                      **FULL)
+    cfg = case["cfg"]
+    nt, n2 = cfg["nt"], cfg["n2"]
+    print(f"grid {cfg['nz']}x{cfg['nx']}, nt {nt}, source rows {case['r_u']}, "
+          f"{int(case['model']['nvel'])} velocity bins", flush=True)
+    assert case["r_u"][0] == n2 - 1
+    seis = _gathers(case, 4, seed=11)
+    ctx = hip_context(case, nshot=4)
+    ctx.set_gathers(seis)
+    engs = [oracle_engine(case, shot=s, nthreads=8) for s in range(4)]
+    ctx.forward_begin()
+    for e in engs:
+        e.forward_begin()
+    k = 2
+    while k < nt:
+        step = min(100, nt - k)
+        ctx.forward_advance(step)
+        for e in engs:
+            e.forward_advance(step)
+        k += step
+        for s, e in enumerate(engs):
+            g, o = ctx.field(hip.FIELD_U, shot=s), e.field(0)
+            assert np.array_equal(g, o), f"step {k} shot {s}: max |diff| {np.max(np.abs(g - o))}"
+        print(f"  forward step {k}: 4 shots equal", flush=True)
+    ctx.backward_begin()
+    for s, e in enumerate(engs):
+        e.backward_begin(seis[s])
+    done = 0
+    while done < nt - 2:
+        step = min(150, nt - 2 - done)
+        ctx.backward_advance(step)
+        for e in engs:
+            e.backward_advance(step)
+        done += step
+        for s, e in enumerate(engs):
+            for which, ow in ((hip.FIELD_Q, 2), (hip.FIELD_I1, 3), (hip.FIELD_I2, 4),
+                              (hip.FIELD_SU, 5), (hip.FIELD_SQ, 6)):
+                g, o = ctx.field(which, shot=s), e.field(ow)
+                assert np.array_equal(g, o), \
+                    f"field {which} shot {s} after {done}: max |diff| {np.max(np.abs(g - o))}"
+            g = ctx.field(hip.FIELD_U, shot=s)[n2:-n2, n2:-n2]
+            assert np.array_equal(g, e.field(0)[n2:-n2, n2:-n2]), f"u shot {s} after {done}"
+        print(f"  backward {done} steps: 4 shots equal", flush=True)
+    assert np.abs(ctx.field(hip.FIELD_I1, shot=1)).max() > 0
+
+
+def test_full_size_properties():
+    from rtm_gpu_amd import hip
+    case = build_case(name="rvsp_full", nt=4000, nshot=1, src_depth=400.0, **FULL)
+    cfg = case["cfg"]
+    assert cfg["nt"] == 4000 and (cfg["nz"], cfg["nx"]) == (411, 1761)
+    ns = 8
+    seis = _gathers(case, ns, seed=12)
+    seis[5] = seis[0]
+    # eight copies of the source; shots 0 and 5 sit in different blocks
+    keys = ("nz", "nx", "n2", "nfdmax", "nt", "nt2", "ilste", "s_l", "s_r", "s_z",
+            "ds", "n", "tao2", "h2", "taoh", "taoh2", "hzx2_1")
+    m = case["model"]
+
+    def context(nshot, icompen):
+        c = hip.Context(**{k: cfg[k] for k in keys}, icompen=icompen, max_shots=nshot)
+        c.set_model(m["v"], m["r1"], m["w"], m["c"], m["index"], m["nvel"], m["vmin"], m["dv"])
+        c.set_wavelet(m["wav"])
+        c.set_shots(case["r_u"] * nshot, case["r_x"] * nshot)
+        return c
+
+    batch = context(ns, 1)
+    batch.set_gathers(seis)
+    batch.migrate()
+    print("batch of 8 migrated", flush=True)
+    i1 = [batch.field(hip.FIELD_I1, shot=s) for s in range(ns)]
+    i2 = [batch.field(hip.FIELD_I2, shot=s) for s in range(ns)]
+    assert all(np.isfinite(a).all() for a in i1 + i2)
+    assert np.abs(i1[0]).max() > 0 and np.abs(i2[0]).max() > 0
+    assert np.array_equal(i1[5], i1[0]), "identical shots in different blocks"
+    for which in (hip.FIELD_Q, hip.FIELD_SQ, hip.FIELD_SU):
+        assert np.array_equal(batch.field(which, shot=5), batch.field(which, shot=0)), which
+    for s in range(1, ns):  # the source side does not see the data
+        assert np.array_equal(i2[s], i2[0]), f"illumination of shot {s}"
+    assert rel_l2(i1[1], i1[0]) > 1e-2  # different data, different image
+
+    single = context(1, 1)
+    single.set_gathers(seis[:1])
+    single.migrate()
+    assert np.array_equal(single.field(hip.FIELD_I1), i1[0]), "batch of 8 vs single shot"
+    assert np.array_equal(single.field(hip.FIELD_I2), i2[0])
+    print("single-shot run equals the batched shot", flush=True)
+
+    # uncompensated imaging: the receiver field starts at zero and is linear
+    # in the data, and every operation is odd under negation.  The one
+    # data-independent term is the initial I1 = spike * u(NT-2) at the source
+    # cell (kernel.cu:859-876), so there I1(d) + I1(-d) = 2 I1(0) up to rounding.
+    nc = context(3, 0)
+    nc.set_gathers(np.stack([seis[0], -seis[0], np.zeros_like(seis[0])]))
+    nc.migrate()
+    a, b, z = (nc.field(hip.FIELD_I1, shot=s) for s in range(3))
+    src = (case["r_u"][0], case["r_x"][0])
+    off = np.ones(a.shape, bool)
+    off[src] = False
+    assert np.count_nonzero(z[off]) == 0  # zero data: only the initial term
+    assert np.abs(a).max() > 0
+    assert np.array_equal(b[off], -a[off]), \
+        f"negated gathers: max |I1(-d) + I1(d)| {np.max(np.abs(a + b)[off])}"
+    # 4000 rounded partial sums of +-u*q: equal up to their rounding
+    assert abs(float(a[src]) + float(b[src]) - 2.0 * float(z[src])) <= \
+        1e-3 * (abs(float(a[src])) + abs(float(b[src])) + abs(float(z[src])))
+    for s in (1, 2):
+        assert np.array_equal(nc.field(hip.FIELD_I2, shot=s), nc.field(hip.FIELD_I2, shot=0))
+    print("negated gathers give the negated image", flush=True)
