@@ -57,3 +57,41 @@ def test_grouped_cgls_on_panels_equals_single_runs():
         for a, b in zip(hb, hi):
             assert abs(a["rr"][i] - b["rr"]) <= 1e-12 * hi[0]["rr"], i
         assert float(np.abs(mb[i] - mi).max()) <= 1e-5 * float(np.abs(mi).max()), i
+
+
+def test_plain_migration_is_the_fallback():
+    """deblur() returns the network's estimate only when its Born re-modelled
+    data fit the observations better than those of the scaled migrated image."""
+    from rtm_gpu_amd import cnn, deblur
+    S = _survey()
+    ctx = deblur.context(S)
+    m_true = S["m_true"]
+    ctx.set_reflectivity(m_true)
+    dobs = ctx.born()
+    ctx.forward_begin(True)
+    ctx.forward_advance(ctx.p.nt)
+    img = ctx.adjoint(dobs)
+    c = float(np.sum(img.astype(np.float64) * m_true) / np.sum(img.astype(np.float64) ** 2))
+    illum = deblur.illumination(S)
+    mnz, mnx = img.shape
+
+    class Exact:  # stands in for a network that recovers the reflectivity
+        def forward(self, x):
+            return np.asarray(m_true, np.float32)[None]
+
+    m, which, r_net, r_rtm = deblur.deblur(ctx, Exact(), S, img, c, illum, dobs)
+    assert which == "net" and r_net == 0.0 < r_rtm and np.array_equal(m, m_true)
+
+    zero = cnn.Deblur([3, 8, 1], mnz * mnx, mnz)
+    zero.init_params(seed=1)  # last layer zero: the output is the scaled image
+    m, which, r_net, r_rtm = deblur.deblur(ctx, zero, S, img, c, illum, dobs)
+    assert which == "rtm" and r_net == r_rtm
+    assert np.array_equal(m, np.float32(c) * img)
+
+    bad = cnn.Deblur([3, 8, 1], mnz * mnx, mnz)
+    p = bad.init_params(seed=2, last_zero=False)
+    bad.set_params(p * 50)
+    m, which, r_net, r_rtm = deblur.deblur(ctx, bad, S, img, c, illum, dobs)
+    print(f"misfit: scaled image {r_rtm:.3f}, inflated network {r_net:.3f}")
+    assert which == "rtm" and not r_net < r_rtm
+    assert np.array_equal(m, np.float32(c) * img)
