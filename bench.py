@@ -396,10 +396,15 @@ def run_deblur():
                "network": (round(res["t_image_s"] + res["t_net_s"], 4)
                            if res["net_within_5pct"] else None)},
            "rel_l2_scaled_image_vs_cgls": round(res["rel_rtm_vs_cg"], 4),
+           # with the run-time fallback: network + two Born passes to pick the
+           # estimate whose re-modelled data fit better
+           "returned": res["chosen"],
+           "speedup_with_fallback_check": round(res["speedup_guarded_vs_cg"], 3),
            "data_residual": {"scaled_image": round(res["resid_rtm"], 4),
                              "network": round(res["resid_net"], 4),
                              "cgls": round(res["resid_cg"], 4)},
            "times_s": {"image": round(res["t_image_s"], 4), "network": round(res["t_net_s"], 5),
+                       "network_with_fallback_check": round(res["t_guarded_s"], 4),
                        "cgls": round(res["t_cg_s"], 3), "training_pairs": round(res["pairs_s"], 1),
                        "training": round(res["train_s"], 1)}}
     print(json.dumps(out), flush=True)
