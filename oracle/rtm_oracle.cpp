@@ -21,7 +21,12 @@
 // Parity pins (tests/): setup integers and LS design tables against the
 // compiled reference helpers (oracle/_ref); Taylor closed form; the wave
 // engine against closed forms and invariants (linearity, mirror symmetry,
-// time-reversal, homogeneous-medium travel time).
+// time-reversal, homogeneous-medium travel time); the image post and stack
+// against closed forms (Laplacian of a quadratic, mirror edges, whitening
+// constant, mean of constant images); the reference's quirks (mis-indexed ABC
+// velocities, the backward wavelet gate, data replacement only for non-zero
+// samples, the k+1 data lag, the backward initial state) by known answers on
+// tiny grids (tests/test_oracle_quirks.py).
 #include "rtm_oracle.h"
 
 #include <pthread.h>
