@@ -33,7 +33,8 @@ enum {
   RTM_E_ARG = -1,     /* bad argument or call order */
   RTM_E_HIP = -2,     /* HIP runtime error */
   RTM_E_NOMEM = -3,   /* device allocation failed */
-  RTM_E_STATE = -4    /* entry point called in the wrong state */
+  RTM_E_STATE = -4,   /* entry point called in the wrong state */
+  RTM_E_BLOWUP = -5   /* a wavefield holds a NaN or an infinity (see check_every) */
 };
 
 enum { RTM_STRIP_F32 = 0, RTM_STRIP_F16 = 1, RTM_STRIP_BF16 = 2 };
@@ -80,6 +81,12 @@ typedef struct rtm_params {
                             the reconstructed u(k) by its snapshot */
   int stack_nx;          /* columns of the stacked image (0 = nx - 2*n2); a ragged batch
                             (rtm_set_shot_geometry) stacks each shot at its own offset */
+  int check_every;       /* not in the reference: every check_every steps (and at the
+                            last step) the newest wavefields are checked for NaN and
+                            infinity, and the *_advance call (or rtm_migrate) that ran
+                            such a step returns RTM_E_BLOWUP; 0 = every 100 steps,
+                            < 0 = never.  A time step beyond the stability limit of the
+                            model (rtmh_stability) blows up this way. */
 } rtm_params;
 
 /* One shot of a ragged batch (uneven apertures; not in the reference, whose
@@ -116,7 +123,9 @@ int rtm_set_wavelet(rtm_ctx* ctx, const float* wav);
 int rtm_set_shots(rtm_ctx* ctx, int nshot, const int* r_u, const int* r_x);
 /* A ragged batch: shot i has its own grid width, receivers, source and image
  * offset.  No counterpart in the reference (a drop-in caller keeps
- * rtm_set_shots).  All shots run in the same launches. */
+ * rtm_set_shots).  All shots run in the same launches.  Model windows set by
+ * rtm_set_shot_model for an earlier batch are dropped: each shot of the new
+ * batch needs its window again before a pass begins (RTM_E_STATE otherwise). */
 int rtm_set_shot_geometry(rtm_ctx* ctx, int nshot, const rtm_shot_geom* geom);
 /* Model window of one shot of a ragged batch: v, r1 [nz*geom.nx] as velocity()
  * builds them for that window.  Call after rtm_set_model (which fixes the
@@ -218,6 +227,7 @@ int rtm_data_dot(rtm_ctx* ctx, int x, int y, int shot0, int nshot, double* out);
 /* gathers = gathers + a*records over shots [shot0, shot0 + nshot) (e.g. the
  * residual update r -= alpha*q). */
 int rtm_data_axpy(rtm_ctx* ctx, float a, int shot0, int nshot);
+
 
 /* Autotuner (not in the reference): for each interior kernel of the
  * migration (forward, source reconstruction, receiver step + imaging) times

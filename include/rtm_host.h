@@ -68,6 +68,14 @@ int rtmh_design_ls(int nthita, int nfdmax, int nfdmin, int nvel, double tao, dou
                    double eps, double fmax, double vmin, double dv, const int* need, double hzx,
                    int* M, int* index, float* c, int c_cap, int nthreads);
 void rtmh_taylor(int nfdmax, float* c /* nfdmax+1 */);
+/* Stability of the time stepping (not in the reference, which runs any time
+ * step): the largest value over the cells v[nxz] of
+ * v^2*tao2*h2*(1 + hzx2_1)*max_k S(k)/4, S(k) = -(c0 + 2 sum_l c_l cos(l k)) the
+ * symbol of the cell's coefficient row (c, index[nvel+1] as for rtm_set_model;
+ * index = NULL: the Taylor row c[nfdmax+1]).  The interior update is stable
+ * when it is <= 1; rtmh_run refuses a model whose number exceeds 1. */
+double rtmh_stability(const float* v, long long nxz, const float* c, const int* index, int nvel,
+                      float vmin, float dv, int nfdmax, float tao2, float h2, float hzx2_1);
 
 /* SEG-Y (SGYWrite.cpp:3-55 with the template of SGY_Model.sgy; segy.cpp codecs). */
 void rtmh_float_to_ibm(const float* in, unsigned char* out_be, long long n);
@@ -116,6 +124,14 @@ typedef struct rtmh_run_opts {
   int resume;          /* 1: skip shots with existing per-shot images */
   int verbose;
   const char* sgy_template; /* path of SGY_Model.sgy (NULL: ./SGY_Model.sgy) */
+  /* Called by a worker before each attempt of a batch, with the batch's
+   * 0-based source indices and the attempt number (0 first); a non-zero
+   * return fails that attempt, which is then re-queued like a failed device
+   * batch (up to `retries` times).  NULL: none.  For progress reports,
+   * cancellation and tests of the re-queue path; it may be called from
+   * several worker threads at once. */
+  int (*before_batch)(const int* shots, int nshots, int attempt, void* user);
+  void* user;
 } rtmh_run_opts;
 int rtmh_run(const char* rtm_txt, const rtmh_run_opts* opts);
 

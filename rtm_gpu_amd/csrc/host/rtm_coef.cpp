@@ -309,4 +309,40 @@ void rtmh_taylor(int nfdmax, float* c) {  // LSMOrCon_rec_2D.cpp:526-551, N = 2*
   }
 }
 
+// Stability of the time stepping (not in the reference).  A Fourier mode
+// exp(i(kx x + kz z)) turns the stencil sum w1 of a cell into
+// -(S(kx) + rho S(kz)) times the mode, S(k) = -(c0 + 2 sum_l c_l cos(l k)), so
+// the leapfrog update u2 = 2u1 - u0 + P w1 has amplification factors g with
+// g + 1/g = 2 - P (S(kx) + rho S(kz)); |g| = 1 for every mode exactly when
+// P (S(kx) + rho S(kz)) <= 4, i.e. P (1 + rho) max_k S(k) / 4 <= 1 (frozen
+// coefficients, cell by cell).  S is sampled on 4096 + 1 points of [0, pi].
+double rtmh_stability(const float* v, long long nxz, const float* c, const int* index, int nvel,
+                      float vmin, float dv, int nfdmax, float tao2, float h2, float hzx2_1) {
+  constexpr int NK = 4096;
+  auto smax_of = [&](const float* row, int taps) {
+    double best = 0.0;
+    for (int j = 0; j <= NK; ++j) {
+      const double k = M_PI * j / NK;
+      double sum = row[0];
+      for (int l = 1; l <= taps; ++l) sum += 2.0 * row[l] * std::cos(l * k);
+      best = std::max(best, -sum);
+    }
+    return best;
+  };
+  std::vector<double> smax(index ? (size_t)nvel : 1, -1.0);
+  double worst = 0.0;
+  for (long long i = 0; i < nxz; ++i) {
+    size_t b = 0;
+    if (index) {  // the cell's bin, as the kernels select it (kernel.cu:55-56)
+      int kb = (int)((double)((v[i] - vmin) / dv) + 0.5);
+      b = (size_t)std::min(std::max(kb, 0), nvel - 1);
+    }
+    if (smax[b] < 0.0)
+      smax[b] = index ? smax_of(c + index[b], index[b + 1] - index[b] - 1) : smax_of(c, nfdmax);
+    const double p = (double)v[i] * v[i] * tao2 * h2;
+    worst = std::max(worst, p * (1.0 + hzx2_1) * smax[b] / 4.0);
+  }
+  return worst;
+}
+
 }  // extern "C"

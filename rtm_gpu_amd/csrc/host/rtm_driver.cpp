@@ -10,13 +10,14 @@
 // kernel.cu:951-990 does.  Those files are the checkpoints: a rerun skips
 // shots whose two files exist.  A failed batch is re-queued up to `retries`
 // times.  Finally the per-shot images are stacked in shot order
-// (kernel.cu:1000-1059), cropped (kernel.cu:1061-1108) and written as SEG-Y
-// (kernel.cu:1181-1209).  The depth-to-time / phase-rotation / time-to-depth
-// side files of kernel.cu:1110-1179 are not produced.
+// (kernel.cu:1000-1059), cropped (kernel.cu:1061-1108), taken through the
+// depth-to-time / phase-rotation / time-to-depth chain into its three side
+// files (kernel.cu:1110-1179) and written as SEG-Y (kernel.cu:1181-1209).
 #include <pthread.h>
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -148,7 +149,13 @@ void* worker(void* arg) {
       b = J.queue.front();
       J.queue.pop_front();
     }
-    rc = run_batch(J, ctx, b);
+    rc = 0;
+    if (J.opt.before_batch) {
+      const int hr = J.opt.before_batch(b.shots.data(), (int)b.shots.size(), b.attempts, J.opt.user);
+      if (hr) rc = set_error(rtmh::RTMH_E_REFUSED, "attempt %d of the batch from shot %d refused (%d)",
+                             b.attempts, b.shots[0] + 1, hr);
+    }
+    if (!rc) rc = run_batch(J, ctx, b);
     std::lock_guard<std::mutex> g(J.mu);
     if (rc) {
       if (++b.attempts <= J.opt.retries) {
@@ -170,7 +177,7 @@ void* worker(void* arg) {
 
 extern "C" int rtmh_run(const char* rtm_txt, const rtmh_run_opts* opts) {
   Job J;
-  J.opt = opts ? *opts : rtmh_run_opts{1, 8, 1, 0, 1, 0, nullptr};
+  J.opt = opts ? *opts : rtmh_run_opts{1, 8, 1, 0, 1, 0, nullptr, nullptr, nullptr};
   if (J.opt.workers < 1) J.opt.workers = 1;
   if (J.opt.batch < 1) J.opt.batch = 1;
   int rc = rtmh_read_config(rtm_txt, &J.cfg);
@@ -205,6 +212,16 @@ extern "C" int rtmh_run(const char* rtm_txt, const rtmh_run_opts* opts) {
   } else {
     J.c.resize(c.nfdmax + 1);
     rtmh_taylor(c.nfdmax, J.c.data());
+  }
+  {  // the reference runs any time step; an unstable one only yields NaN images
+    const double st = rtmh_stability(J.v.data(), (long long)nxz, J.c.data(),
+                                     c.ilste == 0 ? J.index.data() : nullptr, J.nvel, J.vmin, c.dv,
+                                     c.nfdmax, J.d.tao2, J.d.h2, J.d.hzx2_1);
+    if (st > 1.0)
+      return set_error(rtmh::RTMH_E_UNSTABLE,
+                       "time step %g s exceeds the stability limit of the model (number %.4f > 1; "
+                       "the largest stable step is about %g s)", (double)c.tao, st,
+                       (double)c.tao / std::sqrt(st));
   }
   // batches of shots still to do
   Batch cur;

@@ -79,6 +79,12 @@ struct rtm_ctx {
   float* d_grad = nullptr;      // [mnz][gw]
   float* d_stage = nullptr;     // one shot's trace-major gather [n][nt] (host transfers)
   double* d_dot = nullptr;      // FP64 dot-product scratch
+  // finite-value watch: every check_every steps the newest fields are checked
+  unsigned* d_flag = nullptr;   // 0xffffffff, or the step of the first non-finite check
+  unsigned* h_flag = nullptr;   // pinned host copy
+  int check_every = 100;
+  bool checked = false;         // checks launched since the flag was last read
+  bool capturing = false;       // inside rtm_migrate's graph capture
   bool model_ok = false, wav_ok = false, shots_ok = false, data_ok = false, save_strips = true;
   int zs = 32, nxt = 1;
   int zs_epi[16] = {0};  // rows per wave per interior kernel kind (0: zs)
@@ -236,6 +242,35 @@ void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
   a.ua = a.ub = a.uc = nullptr;
 }
 
+// ---- finite-value watch -------------------------------------------------------
+int watch_reset(rtm_ctx* c) {
+  HIPCHK(c, launch_flag_reset(c->d_flag, c->stream));
+  c->checked = false;
+  return RTM_OK;
+}
+bool watch_due(const rtm_ctx* c, int k, int last) {
+  return c->check_every > 0 && (k % c->check_every == 0 || k == last);
+}
+int watch(rtm_ctx* c, const float* f, int k) {
+  HIPCHK(c, launch_finite_check(f, (long long)c->nshot * c->g.fstride, k, c->d_flag, c->stream));
+  c->checked = true;
+  return RTM_OK;
+}
+// Read the flag after the checks of an *_advance call (not while capturing:
+// rtm_migrate reads it after the graph launch).
+int watch_read(rtm_ctx* c, const char* pass) {
+  if (!c->checked || c->capturing) return RTM_OK;
+  HIPCHK(c, hipMemcpyAsync(c->h_flag, c->d_flag, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->checked = false;
+  if (*c->h_flag != 0xffffffffu)
+    return fail(c, RTM_E_BLOWUP,
+                "%s pass: a wavefield holds NaN or infinity at step %u (checked every %d steps); "
+                "the time step may exceed the stability limit of the model, or the inputs are "
+                "not finite", pass, *c->h_flag, c->check_every);
+  return RTM_OK;
+}
+
 int forward_step(rtm_ctx* c, int k) {
   const float wv = (k < c->p.nt2) ? c->wav[k - 1] : 0.0f;  // kernel.cu:812-813
   const bool quant = c->save_strips && c->st.fmt != RTM_STRIP_F32;
@@ -264,6 +299,10 @@ int forward_step(rtm_ctx* c, int k) {
     HIPCHK(c, launch_snap_save(c->g, c->F[c->f2], c->sn, c->d_geo, c->snap_slot[k], c->nshot, c->stream));
   if (c->p.record && !c->born)  // receiver recording of u(k)
     HIPCHK(c, launch_record(c->g, c->r, c->d_geo, c->F[c->f2], c->d_rec, k, c->p.nt, c->nshot, c->stream));
+  if (watch_due(c, k, c->p.nt - 1)) {
+    int rc = watch(c, c->F[c->f2], k);
+    if (rc != RTM_OK) return rc;
+  }
   int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t;  // Deliver (kernel.cu:820)
   return RTM_OK;
 }
@@ -312,6 +351,11 @@ int backward_step(rtm_ctx* c, int k) {
     Timed t(c, RTM_K_RCV_FRAME);
     HIPCHK(c, launch_frame(FR_RCV, c->nfd, a, c->nshot, c->ntiles, c->stream));
   }
+  if (watch_due(c, k, 0)) {  // the reconstructed u(k) and the new receiver field
+    int rc = watch(c, c->F[c->f1], k);
+    if (rc == RTM_OK) rc = watch(c, c->Q[c->q2], k);
+    if (rc != RTM_OK) return rc;
+  }
   { int t = c->q0; c->q0 = c->q1; c->q1 = c->q2; c->q2 = t; }  // Deliver (kernel.cu:921)
   return RTM_OK;
 }
@@ -342,6 +386,10 @@ int born_step(rtm_ctx* c, int k) {
     HIPCHK(c, launch_frame(FR_BORN, c->nfd, a, c->nshot, c->ntiles, c->stream));
   }
   HIPCHK(c, launch_record(c->g, c->r, c->d_geo, c->Q[c->q2], c->d_rec, k, c->p.nt, c->nshot, c->stream));
+  if (watch_due(c, k, c->p.nt - 1)) {
+    rc = watch(c, c->Q[c->q2], k);
+    if (rc != RTM_OK) return rc;
+  }
   int t = c->q0; c->q0 = c->q1; c->q1 = c->q2; c->q2 = t;
   return RTM_OK;
 }
@@ -368,6 +416,10 @@ int adjoint_update(rtm_ctx* c, int k, bool image) {
     a.wav = 0.f; a.k = k;
     Timed t(c, RTM_K_ADJ_FRAME);
     HIPCHK(c, launch_frame(FR_ADJ, c->nfd, a, c->nshot, c->ntiles, c->stream));
+  }
+  if (image && watch_due(c, k, 0)) {
+    int rc = watch(c, c->Q[c->q2], k);
+    if (rc != RTM_OK) return rc;
   }
   int t = c->q0; c->q0 = c->q1; c->q1 = c->q2; c->q2 = t;
   return RTM_OK;
@@ -522,6 +574,7 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
   else if (p->strip_format < 0 || p->strip_format > 2) rc = fail(c, RTM_E_ARG, "strip_format");
   else if (p->snap_every < 0) rc = fail(c, RTM_E_ARG, "snap_every=%d < 0", p->snap_every);
   else if (p->stack_nx < 0) rc = fail(c, RTM_E_ARG, "stack_nx=%d < 0", p->stack_nx);
+  c->check_every = p->check_every == 0 ? 100 : p->check_every;
   if (rc != RTM_OK) {
     int code = rc;
     delete c;
@@ -603,6 +656,11 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_s1, std::max(nS, nI));
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_s2, std::max(nS, nI));
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_imax, (size_t)p->max_shots);
+  if (rc == RTM_OK) rc = dalloc_t(c, &c->d_flag, 1);
+  if (rc == RTM_OK && hipHostMalloc(reinterpret_cast<void**>(&c->h_flag), sizeof(unsigned)) != hipSuccess) {
+    c->h_flag = nullptr;
+    rc = fail(c, RTM_E_NOMEM, "hipHostMalloc of the watch flag failed");
+  }
   {
     std::vector<int4> tiles;  // the widest grid has the most frame tiles
     build_tiles(g, g.nx, 0, tiles);
@@ -637,6 +695,7 @@ int rtm_destroy(rtm_ctx* c) {
   for (auto& ep : c->pending) { (void)hipEventDestroy(ep.a); (void)hipEventDestroy(ep.b); }
   for (auto e : c->pool) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->h_flag) (void)hipHostFree(c->h_flag);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return RTM_OK;
@@ -744,6 +803,10 @@ int rtm_set_shot_geometry(rtm_ctx* c, int nshot, const rtm_shot_geom* sg) {
     geo[i] = ShotGeo{q.nx, mnx, q.s_l, q.s_r, q.n, q.r_u, q.r_x, q.x_img};
   }
   c->geo = geo;
+  // Model windows were laid out for the previous batch's widths: once the
+  // context holds per-shot windows, every shot of the new batch needs its
+  // own again (rtm_forward_begin / rtm_born_begin check this).
+  std::fill(c->shot_model.begin(), c->shot_model.end(), 0);
   return upload_geometry(c);
 }
 
@@ -816,6 +879,8 @@ int rtm_forward_begin(rtm_ctx* c, int save_strips) {
   c->save_strips = save_strips != 0;
   c->born = false;
   const float spike = (float)((double)c->wav[0] / 2.0);  // f(0.0,f0)/2.0 (kernel.cu:803)
+  int wrc = watch_reset(c);
+  if (wrc != RTM_OK) return wrc;
   HIPCHK(c, launch_fwd_begin(c->g, c->F[0], c->F[1], c->d_geo, spike, c->nshot, c->stream));
   if (c->save_strips) {  // Equal (kernel.cu:808)
     if (c->st.fmt != RTM_STRIP_F32)
@@ -843,7 +908,7 @@ int rtm_forward_advance(rtm_ctx* c, int nsteps) {
     int rc = forward_step(c, c->kf);
     if (rc != RTM_OK) return rc;
   }
-  return RTM_OK;
+  return watch_read(c, "forward");
 }
 
 int rtm_backward_begin(rtm_ctx* c) {
@@ -856,6 +921,8 @@ int rtm_backward_begin(rtm_ctx* c) {
   c->q0 = 0; c->q1 = 1; c->q2 = 2;
   c->adj = false;
   const float spike = (float)((double)c->wav[0] / 2.0);
+  int wrc = watch_reset(c);
+  if (wrc != RTM_OK) return wrc;
   HIPCHK(c, launch_bwd_begin(c->g, c->F[c->f0], c->F[c->f1], c->Q[0], c->Q[1], c->SU, c->SQ,
                              c->I1, c->I2, c->d_geo, spike, c->p.icompen, c->nshot, c->stream));
   // BKEqual of the first backward step (kernel.cu:892, k = NT-3): slot NT-2.
@@ -871,7 +938,7 @@ int rtm_backward_advance(rtm_ctx* c, int nsteps) {
     int rc = backward_step(c, c->kb);
     if (rc != RTM_OK) return rc;
   }
-  return RTM_OK;
+  return watch_read(c, "backward");
 }
 
 int rtm_set_reflectivity(rtm_ctx* c, const float* m) {
@@ -929,7 +996,7 @@ int rtm_born_advance(rtm_ctx* c, int nsteps) {
     int rc = born_step(c, c->kf);
     if (rc != RTM_OK) return rc;
   }
-  return RTM_OK;
+  return watch_read(c, "Born");
 }
 
 int rtm_adjoint_begin(rtm_ctx* c) {
@@ -938,6 +1005,8 @@ int rtm_adjoint_begin(rtm_ctx* c) {
   if (!c->save_strips) return fail(c, RTM_E_STATE, "forward pass ran without strips");
   if (!c->data_ok) return fail(c, RTM_E_STATE, "rtm_set_gathers first (residuals)");
   std::swap(c->f0, c->f1);  // f1 = u(nt-2), f0 = u(nt-1), as for the migration
+  int wrc = watch_reset(c);
+  if (wrc != RTM_OK) return wrc;
   const long long fb = (long long)c->nshot * c->g.fstride;
   for (int i = 0; i < 3; ++i) HIPCHK(c, launch_fill(c->Q[i], fb, 0.f, c->stream));
   HIPCHK(c, launch_fill(c->I1, fb, 0.f, c->stream));
@@ -961,7 +1030,7 @@ int rtm_adjoint_advance(rtm_ctx* c, int nsteps) {
     int rc = adjoint_step(c, c->kb);
     if (rc != RTM_OK) return rc;
   }
-  return RTM_OK;
+  return watch_read(c, "adjoint");
 }
 
 int rtm_adjoint_gradient(rtm_ctx* c, float* grad) {
@@ -1002,12 +1071,12 @@ int rtm_data_dot(rtm_ctx* c, int x, int y, int shot0, int nshot, double* out) {
   int rc = data_range(c, shot0, nshot, &off, &n);
   if (rc != RTM_OK) return rc;
   if (!c->d_dot) {
-    rc = dalloc_t(c, &c->d_dot, (size_t)dot64_scratch());
+    rc = dalloc_t(c, &c->d_dot, (size_t)dot64_part_size() + 1);
     if (rc != RTM_OK) return rc;
   }
-  HIPCHK(c, launch_dot64(v[x] + off, v[y] + off, n, c->d_dot, c->stream));
-  HIPCHK(c, hipMemcpyAsync(out, c->d_dot + dot64_scratch() - 1, sizeof(double),
-                           hipMemcpyDeviceToHost, c->stream));
+  double* res = c->d_dot + dot64_part_size();
+  HIPCHK(c, launch_dot64(v[x] + off, v[y] + off, n, 0, 1, c->d_dot, res, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return RTM_OK;
 }
@@ -1042,7 +1111,9 @@ int rtm_migrate(rtm_ctx* c) {
     const int saved_timing = c->timing;
     c->timing = 0;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
     int rc = migrate_eager(c);
+    c->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->timing = saved_timing;
@@ -1068,7 +1139,8 @@ int rtm_migrate(rtm_ctx* c) {
   c->kf = c->g_kf;
   c->kb = c->g_kb;
   c->save_strips = true;
-  return RTM_OK;
+  c->checked = c->check_every > 0;  // the graph holds the checks of both passes
+  return watch_read(c, "migration");
 }
 
 int rtm_set_graph(rtm_ctx* c, int on) {

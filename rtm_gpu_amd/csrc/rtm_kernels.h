@@ -126,6 +126,12 @@ hipError_t launch_interior(int epi, int nfd_inst, IntArgs a, int nshot, hipStrea
 hipError_t launch_frame(int mode, int nfd_inst, FrameArgs a, int nshot, int nrecords,
                         hipStream_t s);
 hipError_t launch_fill(float* p, long long n, float val, hipStream_t s);
+// Finite-value watch: reset *flag to 0xffffffff; store k in *flag (if still
+// 0xffffffff) when any of the n floats at f (n a multiple of 4, 16-byte
+// aligned) is a NaN or an infinity.
+hipError_t launch_flag_reset(unsigned* flag, hipStream_t s);
+hipError_t launch_finite_check(const float* f, long long n, int k, unsigned* flag, hipStream_t s);
+
 // Per-shot kernels take the batch's geometry array; launches cover shots
 // 0..nshot-1 (blockIdx.y).
 hipError_t launch_fwd_begin(const Grid& g, float* f0, float* f1, const ShotGeo* geo, float spike,
@@ -163,11 +169,13 @@ hipError_t launch_refl_scatter(const Grid& g, const ShotGeo* geo, int gw, const 
 hipError_t launch_grad_stack(const Grid& g, const ShotGeo* geo, int gw, const float* G,
                              const float* pm, long long mstride, int nshot, float* grad,
                              hipStream_t s);
-// FP64 dot product of float arrays a, b [n]: scratch holds dot64_scratch()
-// doubles; the result lands in scratch[dot64_scratch() - 1].
-int dot64_scratch();
-hipError_t launch_dot64(const float* a, const float* b, long long n, double* scratch,
-                        hipStream_t s);
+// FP64 dot products of nvec pairs of float vectors of n elements, vector v at
+// a + v*stride, b + v*stride: part holds nvec*dot64_part_size() doubles of
+// partial sums, out[v] receives the dot product of pair v.  The reduction
+// order is fixed and the same for every vector.
+int dot64_part_size();
+hipError_t launch_dot64(const float* a, const float* b, long long n, long long stride, int nvec,
+                        double* part, double* out, hipStream_t s);
 // y = y + a*x over n elements
 hipError_t launch_axpy(float* y, const float* x, float a, long long n, hipStream_t s);
 // out[c][r] = in[r][c] (rows x cols, row-major)

@@ -681,6 +681,24 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Finite-value watch (not in the reference): *flag stays 0xffffffff while every
+// checked field is finite; the first check that meets a NaN or an infinity
+// stores its step k there.
+__global__ void k_flag_reset(unsigned* flag) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 0xffffffffu;
+}
+__global__ __launch_bounds__(256) void k_finite_check(const float4* f, long long n4, int k,
+                                                      unsigned* flag) {
+  bool bad = false;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float4 v = f[i];
+    bad = bad || !(isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w));
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicCAS(flag, 0xffffffffu, (unsigned)k);
+}
+
+// ---------------------------------------------------------------------------
 // Small kernels.
 __global__ void k_fill(float* p, long long n, float val) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -976,22 +994,29 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   }
   return sh[0];
 }
+// Vector v = blockIdx.y of a batch starts at v*stride in a and b; its
+// partial sums fill row v of part (gridDim.x = DOT_BLOCKS entries), so every
+// vector is reduced in the same order as a single-vector launch.
 __global__ __launch_bounds__(256) void k_dot64_part(const float* a, const float* b, long long n,
-                                                    double* part) {
+                                                    long long stride, double* part) {
   __shared__ double sh[256];
+  a += (long long)blockIdx.y * stride;
+  b += (long long)blockIdx.y * stride;
   double acc = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     acc += (double)a[i] * (double)b[i];
   const double t = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  if (threadIdx.x == 0) part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = t;
 }
+// One block per vector: out[v] = the fixed-tree sum of row v of part.
 __global__ __launch_bounds__(256) void k_dot64_final(const double* part, int np, double* out) {
   __shared__ double sh[256];
+  part += (long long)blockIdx.x * np;
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
   const double t = block_sum(acc, sh);
-  if (threadIdx.x == 0) out[0] = t;
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
 // y = y + a*x (product, then sum)
@@ -1096,6 +1121,16 @@ static inline unsigned blocks_for(long long n, int per = 256, unsigned cap = 409
   return (unsigned)(b > cap ? cap : b);
 }
 
+hipError_t launch_flag_reset(unsigned* flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_flag_reset, dim3(1), dim3(64), 0, s, flag);
+  return hipGetLastError();
+}
+hipError_t launch_finite_check(const float* f, long long n, int k, unsigned* flag, hipStream_t s) {
+  const long long n4 = n / 4;
+  hipLaunchKernelGGL(k_finite_check, dim3(blocks_for(n4, 256, 2048)), dim3(256), 0, s,
+                     reinterpret_cast<const float4*>(f), n4, k, flag);
+  return hipGetLastError();
+}
 hipError_t launch_fill(float* p, long long n, float val, hipStream_t s) {
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, s, p, n, val);
   return hipGetLastError();
@@ -1192,13 +1227,13 @@ hipError_t launch_grad_stack(const Grid& g, const ShotGeo* geo, int gw, const fl
   return hipGetLastError();
 }
 
-int dot64_scratch() { return DOT_BLOCKS + 1; }
+int dot64_part_size() { return DOT_BLOCKS; }
 
-hipError_t launch_dot64(const float* a, const float* b, long long n, double* scratch,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(k_dot64_part, dim3(DOT_BLOCKS), dim3(256), 0, s, a, b, n, scratch);
-  hipLaunchKernelGGL(k_dot64_final, dim3(1), dim3(256), 0, s, scratch, DOT_BLOCKS,
-                     scratch + DOT_BLOCKS);
+hipError_t launch_dot64(const float* a, const float* b, long long n, long long stride, int nvec,
+                        double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot64_part, dim3(DOT_BLOCKS, nvec), dim3(256), 0, s, a, b, n, stride,
+                     part);
+  hipLaunchKernelGGL(k_dot64_final, dim3(nvec), dim3(256), 0, s, part, DOT_BLOCKS, out);
   return hipGetLastError();
 }
 

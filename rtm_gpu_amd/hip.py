@@ -32,7 +32,7 @@ class RtmParams(C.Structure):
                                         "icompen", "s_l", "s_r", "s_z", "ds", "n")] + \
                [(n, C.c_float) for n in ("tao2", "h2", "taoh", "taoh2", "hzx2_1")] + \
                [(n, C.c_int) for n in ("max_shots", "strip_format", "record", "zseg",
-                                       "snap_every", "stack_nx")]
+                                       "snap_every", "stack_nx", "check_every")]
 
 
 class RtmShotGeom(C.Structure):

@@ -33,10 +33,14 @@ class Derived(C.Structure):
                [(n, C.c_float) for n in ("taoh", "tao2", "h2", "taoh2", "hzx", "hzx2_1")]
 
 
+BEFORE_BATCH = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_void_p)
+
+
 class RunOpts(C.Structure):
     _fields_ = [("workers", C.c_int), ("batch", C.c_int), ("retries", C.c_int),
                 ("strip_format", C.c_int), ("resume", C.c_int), ("verbose", C.c_int),
-                ("sgy_template", C.c_char_p)]
+                ("sgy_template", C.c_char_p), ("before_batch", BEFORE_BATCH),
+                ("user", C.c_void_p)]
 
 
 class HostError(RuntimeError):
@@ -76,6 +80,9 @@ def lib():
             [_ip, C.c_double, _ip, _ip, _fp, C.c_int, C.c_int]
         L.rtmh_taylor.argtypes = [C.c_int, _fp]
         L.rtmh_taylor.restype = None
+        L.rtmh_stability.argtypes = [_fp, C.c_longlong, _fp, C.c_void_p, C.c_int, C.c_float,
+                                     C.c_float, C.c_int, C.c_float, C.c_float, C.c_float]
+        L.rtmh_stability.restype = C.c_double
         L.rtmh_float_to_ibm.argtypes = [_fp, _bp, C.c_longlong]
         L.rtmh_float_to_ibm.restype = None
         L.rtmh_ibm_to_float.argtypes = [_bp, _fp, C.c_longlong]
@@ -182,6 +189,19 @@ def taylor(nfdmax):
     return c
 
 
+def stability(v, c, index, nvel, vmin, dv, nfdmax, tao2, h2, hzx2_1):
+    """The time-stepping stability number of a model (rtmh_stability): stable
+    when <= 1.  index None: one Taylor row c[nfdmax+1]."""
+    v = np.ascontiguousarray(v, np.float32).ravel()
+    c = np.ascontiguousarray(c, np.float32).ravel()
+    idx = None
+    if index is not None:
+        idx = np.ascontiguousarray(index, np.int32)
+    return lib().rtmh_stability(v, v.size, c, None if idx is None else idx.ctypes.data,
+                                int(nvel), float(vmin), float(dv), int(nfdmax), float(tao2),
+                                float(h2), float(hzx2_1))
+
+
 def float_to_ibm(x):
     x = np.ascontiguousarray(x, np.float32).ravel()
     out = np.zeros(4 * x.size, np.uint8)
@@ -264,7 +284,15 @@ def phase_rotate(d, angle):
 
 
 def run(rtm_txt, workers=1, batch=8, retries=1, strip_format=0, resume=1, verbose=0,
-        sgy_template=None):
+        sgy_template=None, before_batch=None):
+    """The reference-style driver (rtmh_run).  before_batch(shots, attempt) ->
+    bool, if given, is called before each attempt of a batch (shots: 0-based
+    source indices); returning False fails that attempt, which is re-queued."""
+    cb = BEFORE_BATCH()
+    if before_batch is not None:
+        def _cb(p, n, attempt, _user):
+            return 0 if before_batch([p[i] for i in range(n)], attempt) else 1
+        cb = BEFORE_BATCH(_cb)
     o = RunOpts(workers, batch, retries, strip_format, resume, verbose,
-                None if sgy_template is None else str(sgy_template).encode())
+                None if sgy_template is None else str(sgy_template).encode(), cb, None)
     return _chk(lib().rtmh_run(str(rtm_txt).encode(), C.byref(o)), "rtmh_run")

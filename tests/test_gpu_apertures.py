@@ -118,3 +118,37 @@ def test_ragged_batch_equals_single_shot_runs():
     assert np.array_equal(ill, b)
     assert np.array_equal(img, exp_img.astype(np.float32))
     assert np.isfinite(img).all()
+
+
+def test_new_geometry_drops_model_windows():
+    # A second rtm_set_shot_geometry must not leave the previous batch's model
+    # windows in force: the next pass refuses to begin until each shot of the
+    # new batch has its window again.
+    from rtm_gpu_amd import hip
+    shots, _ = _problem()
+    wmax = max(s["wd"] for s in shots)
+    c0 = shots[0]["case"]
+    base = {k: c0["cfg"][k] for k in _keys()}
+    nxmax = wmax + 2 * N2
+    ctx = hip.Context(**base, nx=nxmax, s_l=N2, s_r=N2 + wmax - 1, n=wmax, max_shots=len(shots),
+                      stack_nx=MOD_NX)
+    m = c0["model"]
+    vflat = np.full((base["nz"], nxmax), m["vmin"], np.float32)
+    ctx.set_model(vflat, vflat * 0, m["w"], m["c"], m["index"], m["nvel"], m["vmin"], m["dv"])
+    ctx.set_wavelet(m["wav"])
+    geoms = []
+    for s in shots:
+        cf = s["case"]["cfg"]
+        geoms.append(dict(nx=cf["nx"], r_u=s["case"]["r_u"][0], r_x=s["case"]["r_x"][0],
+                          s_l=cf["s_l"], s_r=cf["s_r"], n=cf["n"], x_img=s["xb"]))
+    ctx.set_shot_geometry(geoms)
+    for i, s in enumerate(shots):
+        ctx.set_shot_model(i, s["case"]["model"]["v"], s["case"]["model"]["r1"])
+    ctx.forward_begin()
+    ctx.set_shot_geometry(geoms[2:])
+    with pytest.raises(hip.RtmError, match=r"\(-4\).*no model window"):
+        ctx.forward_begin()
+    for i, s in enumerate(shots[2:]):
+        ctx.set_shot_model(i, s["case"]["model"]["v"], s["case"]["model"]["r1"])
+    ctx.forward_begin()
+    ctx.close()
