@@ -228,6 +228,20 @@ int rtm_data_dot(rtm_ctx* ctx, int x, int y, int shot0, int nshot, double* out);
  * residual update r -= alpha*q). */
 int rtm_data_axpy(rtm_ctx* ctx, float a, int shot0, int nshot);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops): the FP64 dot
+ * products of rtm_data_dot on caller-owned device vectors, with no allocation,
+ * no copy to the host and no synchronisation.  out[v] (device memory) receives
+ * sum_i x[v*stride + i] * y[v*stride + i], i < n, for v < nvec, in the same
+ * fixed reduction order as rtm_data_dot, on stream (a hipStream_t; NULL = the
+ * null stream).  The partial sums live in a static device array of
+ * RTM_DOT_NOALLOC_MAX_VEC * 1024 doubles (8 KiB per vector) per GPU, held until
+ * the process ends; nvec beyond RTM_DOT_NOALLOC_MAX_VEC is rejected.  Calls
+ * that overlap on one GPU race on that array.  Errors are reported by
+ * rtm_last_error(NULL) on the calling thread. */
+#define RTM_DOT_NOALLOC_MAX_VEC 64
+int rtm_data_dot_noalloc(const float* x, const float* y, long long n, long long stride, int nvec,
+                         double* out, void* stream);
 
 /* Autotuner (not in the reference): for each interior kernel of the
  * migration (forward, source reconstruction, receiver step + imaging) times

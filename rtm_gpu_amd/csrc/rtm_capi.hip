@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -701,7 +702,10 @@ int rtm_destroy(rtm_ctx* c) {
   return RTM_OK;
 }
 
-const char* rtm_last_error(const rtm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+// Message of the last failed call that takes no context (per host thread).
+static thread_local std::string t_free_err = "null context";
+
+const char* rtm_last_error(const rtm_ctx* c) { return c ? c->err.c_str() : t_free_err.c_str(); }
 size_t rtm_device_bytes(const rtm_ctx* c) { return c ? c->bytes : 0; }
 
 int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, const float* coef,
@@ -1089,6 +1093,73 @@ int rtm_data_axpy(rtm_ctx* c, float a, int shot0, int nshot) {
   int rc = data_range(c, shot0, nshot, &off, &n);
   if (rc != RTM_OK) return rc;
   HIPCHK(c, launch_axpy(c->d_data + off, c->d_rec + off, a, n, c->stream));
+  return RTM_OK;
+}
+
+// ---- FP64 dot products without allocation ---------------------------------
+// The partial sums of rtm_data_dot_noalloc live in a file-scope device array,
+// one copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol):
+// the device address of each GPU's copy comes from hipGetSymbolAddress on the
+// first call on that device and is kept per device.
+static __device__ __align__(256) double scratch[RTM_DOT_NOALLOC_MAX_VEC * 1024];
+static_assert(RTM_DOT_NOALLOC_MAX_VEC >= 1, "scratch size");
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int rtm_data_dot_noalloc(const float* x, const float* y, long long n, long long stride, int nvec,
+                         double* out, void* stream) {
+  if (!x || !y || !out) {
+    t_free_err = "rtm_data_dot_noalloc: null vector or output";
+    return RTM_E_ARG;
+  }
+  if (n < 0 || nvec < 1 || (nvec > 1 && stride < n)) {
+    char b[160];
+    snprintf(b, sizeof(b), "rtm_data_dot_noalloc: n = %lld, nvec = %d, stride = %lld", n, nvec,
+             stride);
+    t_free_err = b;
+    return RTM_E_ARG;
+  }
+  if (nvec > RTM_DOT_NOALLOC_MAX_VEC) {
+    char b[160];
+    snprintf(b, sizeof(b), "rtm_data_dot_noalloc: %d vectors exceed the scratch maximum of %d",
+             nvec, RTM_DOT_NOALLOC_MAX_VEC);
+    t_free_err = b;
+    return RTM_E_ARG;
+  }
+  if (dot64_part_size() * RTM_DOT_NOALLOC_MAX_VEC > (int)(sizeof(scratch) / sizeof(double))) {
+    t_free_err = "rtm_data_dot_noalloc: scratch smaller than the partial sums";
+    return RTM_E_STATE;
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    t_free_err = std::string("rtm_data_dot_noalloc: hipGetDevice: ") + hipGetErrorString(e);
+    return RTM_E_HIP;
+  }
+  double* part = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void* p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        t_free_err = std::string("rtm_data_dot_noalloc: hipGetSymbolAddress: ") +
+                     hipGetErrorString(e);
+        return RTM_E_HIP;
+      }
+      g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    part = g_scratch_ptr[dev];
+  }
+  e = launch_dot64(x, y, n, stride, nvec, part, out, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) {
+    t_free_err = std::string("rtm_data_dot_noalloc: launch: ") + hipGetErrorString(e);
+    return RTM_E_HIP;
+  }
   return RTM_OK;
 }
 

@@ -91,6 +91,8 @@ def lib():
         L.rtm_adjoint_gradient.argtypes = [vp, _fp]
         L.rtm_data_dot.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.rtm_data_axpy.argtypes = [vp, C.c_float, C.c_int, C.c_int]
+        L.rtm_data_dot_noalloc.argtypes = [vp, vp, C.c_longlong, C.c_longlong, C.c_int, vp, vp]
+        L.rtm_data_dot_noalloc.restype = C.c_int
         L.rtm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
@@ -102,6 +104,21 @@ def lib():
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
+
+
+DOT_NOALLOC_MAX_VEC = 64  # RTM_DOT_NOALLOC_MAX_VEC
+
+
+def data_dot_noalloc(x_ptr, y_ptr, n, nvec=1, stride=0, out_ptr=None, stream=None):
+    """FP64 dot products of nvec pairs of device float vectors (addresses
+    x_ptr, y_ptr; vector v at v*stride elements) into the device doubles at
+    out_ptr, on the given HIP stream handle, with no allocation and no
+    synchronisation (rtm_data_dot_noalloc)."""
+    rc = lib().rtm_data_dot_noalloc(C.c_void_p(x_ptr), C.c_void_p(y_ptr), int(n), int(stride),
+                                    int(nvec), C.c_void_p(out_ptr), C.c_void_p(stream))
+    if rc != 0:
+        raise RtmError(f"rtm_data_dot_noalloc failed ({rc}): "
+                       f"{lib().rtm_last_error(None).decode()}")
 
 
 class Context:
