@@ -246,50 +246,147 @@ def algorithmic_bytes(kind, d, n2, nfd, nshot, n):
 
 
 def pmc_traffic(kind_name, workload):
-    """HBM bytes per launch from a committed rocprofv3 PMC summary, if any."""
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, with the
+    summary's own record of when and on which kernel build it was measured;
+    None when there is none for this kernel."""
     p = os.path.join(ROOT, "profiles", "pmc_summary.json")
     if not os.path.exists(p):
-        return None
+        return None, None
     try:
         with open(p) as f:
             s = json.load(f)
         e = s.get(workload, {}).get(kind_name)
         if not e:
-            return None
-        return float(e["hbm_bytes_per_launch"])
+            return None, None
+        src = f"profiles/pmc_summary.json ({s.get('measured', 'date unrecorded')})"
+        return float(e["hbm_bytes_per_launch"]), src
     except (OSError, ValueError, KeyError):
-        return None
+        return None, None
 
 
-def cpu_baseline(P, target_s=15.0):
-    """The FP32 CPU restatement (oracle/) on the host cores: one shot per
-    pthread, a bounded number of forward and backward steps on the full grid."""
+def host_cpus():
+    """Cores this process may use, what the OS reports, and the CPU model."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return allowed, os.cpu_count(), model
+
+
+def _oracle_cfg(P, shot, nthreads, record=0, strip_slots=0):
+    import oracle_bind as ob
+    d, m, cfgp = P["d"], P["models"]["mig"], P["cfg"]
+    return ob.OrcCfg(nz=d.nz, nx=d.nx, n2=cfgp.n2, nfdmax=cfgp.nfdmax, nt=d.nt, nt2=d.nt2,
+                     ilste=0, icompen=1, s_l=d.s_l, s_r=d.s_r, s_z=d.s_z, ds=1, n=cfgp.n,
+                     r_u=P["r_u"][shot], r_x=d.r_x, tao2=d.tao2, h2=d.h2, taoh=d.taoh,
+                     taoh2=d.taoh2, hzx2_1=d.hzx2_1, vmin=m["vmin"], dv=1.0, record=record,
+                     nthreads=nthreads, strip_slots=strip_slots)
+
+
+def cpu_baseline(P, target_s=12.0):
+    """The FP32 CPU restatement (oracle/) on the host cores, in its two forms:
+    throughput (one shot per pthread) and latency (one shot whose every step
+    is split by rows over the pthreads, a barrier per step), each a bounded
+    number of forward and backward steps on the full grid.  The reported
+    value is the throughput form.  Threads: the cores this process may use,
+    at most 16 (one GPU's share of the box's cores)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ctypes as C
     import oracle_bind as ob
     d, m, cfgp = P["d"], P["models"]["mig"], P["cfg"]
-    n2 = cfgp.n2
-    cores = max(1, min(16, os.cpu_count() or 1))
-    cfg = ob.OrcCfg(nz=d.nz, nx=d.nx, n2=n2, nfdmax=cfgp.nfdmax, nt=d.nt, nt2=d.nt2, ilste=0,
-                    icompen=1, s_l=d.s_l, s_r=d.s_r, s_z=d.s_z, ds=1, n=cfgp.n,
-                    r_u=P["r_u"][len(P["r_u"]) // 2], r_x=d.r_x, tao2=d.tao2, h2=d.h2,
-                    taoh=d.taoh, taoh2=d.taoh2, hzx2_1=d.hzx2_1, vmin=m["vmin"], dv=1.0,
-                    record=0, nthreads=1, strip_slots=64)
+    allowed, nproc, model = host_cpus()
+    cores = max(1, min(16, allowed))
     seis = np.zeros(cfgp.n * d.nt, np.float32)
     args = [m["v"].ravel(), m["r1"].ravel(), P["w"], m["c"], len(m["c"]),
             m["index"].ctypes.data, m["nvel"], P["wav"], seis]
-    ob.lib()
+    nI = (d.nz - 2 * cfgp.n2) * (d.nx - 2 * cfgp.n2)
+    per_pair_upd = 2 * d.nz * d.nx + nI
+    # the timing sample keeps 64 strip slots (its backward steps only time the kernels)
+    cfg = _oracle_cfg(P, len(P["r_u"]) // 2, 1, strip_slots=64)
     probe_steps = 3
     t_probe = ob.lib().orc_time_shots(C.byref(cfg), *args, 1, 1, probe_steps, probe_steps)
     per_pair = max(t_probe / probe_steps, 1e-3)
     steps = int(max(4, min(d.nt - 3, target_s / per_pair)))
     el = ob.lib().orc_time_shots(C.byref(cfg), *args, cores, cores, steps, steps)
-    nI = (d.nz - 2 * n2) * (d.nx - 2 * n2)
-    upd = cores * steps * (2 * d.nz * d.nx + nI)
-    return {"value": upd / el / 1e9, "unit": "Gpt-updates/s", "cores": cores, "kind": "port",
+    thr = cores * steps * per_pair_upd / el / 1e9
+    # latency form: one shot, rows split over the threads
+    steps_r = int(max(4, min(d.nt - 3, target_s * cores / per_pair / 4)))
+    el_r = ob.lib().orc_time_rows(C.byref(cfg), *args, cores, steps_r, steps_r)
+    lat = steps_r * per_pair_upd / el_r / 1e9
+    shot_s = el_r / steps_r * (d.nt - 2)
+    return {"value": thr, "unit": "Gpt-updates/s", "cores": cores, "kind": "port",
             "sample": f"FP32 restatement (oracle/), {cores} shots on {cores} pthreads, "
                       f"{steps} forward + {steps} backward steps each on the {d.nz}x{d.nx} "
-                      f"grid, {el:.1f} s"}
+                      f"grid, {el:.1f} s",
+            "row_parallel": {"value": lat, "unit": "Gpt-updates/s", "threads": cores,
+                             "s_per_shot": round(shot_s, 2),
+                             "sample": f"one shot, {steps_r} forward + {steps_r} backward "
+                                       f"steps with rows split over {cores} pthreads, "
+                                       f"{el_r:.1f} s"},
+            "host": {"threads_allowed": allowed, "nproc": nproc, "cpu_model": model}}
+
+
+def correctness(P, nt_check=400):
+    """One shot of the benched batch (the first source, in the ABC at row
+    N2-1) migrated at truncated nt on the full grid, by the GPU and by the
+    oracle (the FP32 restatement as the checker, and the FP64 one for the
+    error): the GPU's image equals the FP32 oracle's bit for bit, and its
+    relative L2 distance to the FP64 image is stated.  Run in the CPU leg."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    from rtm_gpu_amd import hip
+    d, cfgp = P["d"], P["cfg"]
+    mt, mm = P["models"]["true"], P["models"]["mig"]
+    nt = min(nt_check, d.nt)
+    nt2 = min(d.nt2, nt)
+    ctx = hip.Context(nz=d.nz, nx=d.nx, n2=cfgp.n2, nfdmax=cfgp.nfdmax, nt=nt, nt2=nt2,
+                      ilste=0, icompen=1, s_l=d.s_l, s_r=d.s_r, s_z=d.s_z, ds=1, n=cfgp.n,
+                      tao2=d.tao2, h2=d.h2, taoh=d.taoh, taoh2=d.taoh2, hzx2_1=d.hzx2_1,
+                      max_shots=1, record=1)
+    wav = P["wav"][:nt + 1]
+    ctx.set_wavelet(wav)
+    ctx.set_shots(P["r_u"][:1], P["r_x"][:1])
+    ctx.set_model(mt["v"], mt["r1"], P["w"], mt["c"], mt["index"], mt["nvel"], mt["vmin"], 1.0)
+    ctx.forward_begin(save_strips=False)
+    ctx.forward_advance(nt)
+    seis = ctx.records()[0]                       # [n][nt]
+    ctx.gathers_from_records()
+    ctx.set_model(mm["v"], mm["r1"], P["w"], mm["c"], mm["index"], mm["nvel"], mm["vmin"], 1.0)
+    ctx.migrate()
+    gpu = ctx.field(hip.FIELD_I1)
+    ctx.close()
+    cores = max(1, min(16, host_cpus()[0]))
+    res = {}
+    for prec in (32, 64):
+        cfg = _oracle_cfg(P, 0, cores)
+        cfg.nt, cfg.nt2 = nt, nt2
+        fields = {k: getattr(cfg, k) for k, _ in ob.OrcCfg._fields_}
+        model = dict(v=mm["v"], r1=mm["r1"], w=P["w"], c=mm["c"], index=mm["index"],
+                     nvel=mm["nvel"], wav=wav)
+        e = ob.Engine(fields, model, prec=prec)
+        e.forward_begin()
+        e.forward_advance(nt)
+        e.backward_begin(seis)
+        e.backward_advance(nt)
+        res[prec] = e.field(3, f64=(prec == 64))
+    n2 = cfgp.n2
+    g, r32, r64 = (x[n2:-n2, n2:-n2].astype(np.float64) for x in (gpu, res[32], res[64]))
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+    return {"check": f"source 1 (row {P['r_u'][0]}), {nt} steps on the {d.nz}x{d.nx} grid, "
+                     "image I1 over the interior",
+            "image_bitexact_vs_fp32_oracle": bool(np.array_equal(g, r32)),
+            "image_rel_l2_vs_fp32_oracle": rel(g, r32),
+            "image_rel_l2_vs_fp64_oracle": rel(g, r64),
+            "fp32_oracle_rel_l2_vs_fp64": rel(r32, r64)}
 
 
 def copy_probe(mib):
@@ -434,7 +531,7 @@ def copy_bandwidth(gib=2, reps=5):
     a, b, e0, e1 = vp(), vp(), vp(), vp()
     ok = rt.hipMalloc(C.byref(a), nb) == 0 and rt.hipMalloc(C.byref(b), nb) == 0
     ok = ok and rt.hipEventCreate(C.byref(e0)) == 0 and rt.hipEventCreate(C.byref(e1)) == 0
-    ms = C.c_float(0.0)
+    ms, ms_k = C.c_float(0.0), C.c_float(0.0)
     if ok:
         rt.hipMemcpy(b, a, nb, 3)  # device to device
         rt.hipEventRecord(e0, None)
@@ -443,6 +540,16 @@ def copy_bandwidth(gib=2, reps=5):
         rt.hipEventRecord(e1, None)
         rt.hipEventSynchronize(e1)
         rt.hipEventElapsedTime(C.byref(ms), e0, e1)
+        # the hand-written float4 streaming kernel on the same buffers
+        lib = hip.lib()
+        ok_k = lib.rtm_stream_copy(a, b, nb // 4, None) == 0
+        rt.hipEventRecord(e0, None)
+        for _ in range(reps):
+            ok_k = ok_k and lib.rtm_stream_copy(a, b, nb // 4, None) == 0
+        rt.hipEventRecord(e1, None)
+        rt.hipEventSynchronize(e1)
+        if ok_k:
+            rt.hipEventElapsedTime(C.byref(ms_k), e0, e1)
     for p in (a, b):
         if p.value:
             rt.hipFree(p)
@@ -450,8 +557,9 @@ def copy_bandwidth(gib=2, reps=5):
         if e.value:
             rt.hipEventDestroy(e)
     if not ok or ms.value <= 0:
-        return None
-    return 2 * nb / (ms.value / reps * 1e-3) / 1e9
+        return None, None
+    rate = lambda t: 2 * nb / (t / reps * 1e-3) / 1e9 if t > 0 else None  # noqa: E731
+    return rate(ms.value), rate(ms_k.value)
 
 
 def main():
@@ -552,8 +660,9 @@ def main():
         log(f"[bench] step {i + 1}/{a.steps} done at {time.perf_counter() - t0:.2f}s")
     ctx.sync()
     el = time.perf_counter() - t0
-    copy_gbs = copy_bandwidth()
-    log(f"[bench] device copy right after the timed steps: {copy_gbs or 0:.0f} GB/s")
+    copy_gbs, stream_gbs = copy_bandwidth()
+    log(f"[bench] device copy right after the timed steps: {copy_gbs or 0:.0f} GB/s (runtime), "
+        f"{stream_gbs or 0:.0f} GB/s (float4 streaming kernel)")
     img, _ = ctx.stack_finish(a.shots * (a.warmup + a.steps), 1)
     assert np.isfinite(img).all(), "non-finite image"
 
@@ -575,6 +684,7 @@ def main():
     kd = kinds[dom]
     workload = f"rvsp_faulted_351x1701_nt{d.nt}_{a.shots}shots"
     total_bytes = sum(v["bytes_per_launch"] * v["launches"] for v in kinds.values())
+    traffic, traffic_src = pmc_traffic(dom, f"rvsp_faulted_351x1701_{a.shots}shots")
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -599,19 +709,25 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(kd["gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kd["gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic(dom, f"rvsp_faulted_351x1701_{a.shots}shots"),
+                     "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": kd["bytes_per_launch"],
                      "avg_launch_ms": round(kd["avg_ms"], 5),
                      "copy_GBps_same_run": round(copy_gbs, 1) if copy_gbs else None,
+                     "stream_kernel_GBps_same_run": round(stream_gbs, 1) if stream_gbs else None,
                      "frac_of_copy": round(kd["gbs"] / copy_gbs, 4) if copy_gbs else None},
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "launches": v["launches"],
                         "GB/s": round(v["gbs"], 1)} for k, v in kinds.items()},
     }
     ctx.close()
     if not a.no_cpu:
+        log("[bench] correctness check (one shot, truncated nt, against the oracle) ...")
+        out["correctness"] = correctness(P)
+        log(f"[bench] correctness: {out['correctness']}")
         log("[bench] CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(P)
     else:
+        out["correctness"] = None
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 

@@ -239,6 +239,12 @@ int rtm_data_axpy(rtm_ctx* ctx, float a, int shot0, int nshot);
  * the process ends; nvec beyond RTM_DOT_NOALLOC_MAX_VEC is rejected.  Calls
  * that overlap on one GPU race on that array.  Errors are reported by
  * rtm_last_error(NULL) on the calling thread. */
+/* No counterpart in the reference; a measurement aid.  dst[i] = src[i] for
+ * i < n (device memory, 16-byte aligned, n a multiple of 4) with a plain
+ * float4 streaming kernel on stream (NULL = the null stream): the bandwidth a
+ * pure streaming kernel reaches on this device. */
+int rtm_stream_copy(const float* src, float* dst, long long n, void* stream);
+
 #define RTM_DOT_NOALLOC_MAX_VEC 64
 int rtm_data_dot_noalloc(const float* x, const float* y, long long n, long long stride, int nvec,
                          double* out, void* stream);

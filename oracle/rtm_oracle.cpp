@@ -1158,11 +1158,12 @@ struct TimeJob {
   const float *v, *r1, *w, *c, *wav, *seis;
   const int* index;
   int nc, nvel, nsteps_f, nsteps_b;
+  int row_threads;  // threads of the engine's row pool (1: one shot per thread)
 };
 static void* time_worker(void* p) {
   TimeJob* j = (TimeJob*)p;
   orc_cfg cfg = j->cfg;
-  cfg.nthreads = 1;
+  cfg.nthreads = j->row_threads;
   Engine<float> e(cfg);
   set_model_t(&e, j->v, j->r1, j->w, j->c, j->nc, j->index, j->nvel, j->wav);
   e.forward_begin();
@@ -1180,7 +1181,7 @@ double orc_time_shots(const orc_cfg* cfg, const float* v, const float* r1, const
                       const float* c, int nc, const int* index, int nvel, const float* wav,
                       const float* seis, int nshot, int nthreads, int nsteps_f, int nsteps_b) {
   std::vector<TimeJob> jobs(nshot, TimeJob{*cfg, v, r1, w, c, wav, seis, index, nc, nvel,
-                                           nsteps_f, nsteps_b});
+                                           nsteps_f, nsteps_b, 1});
   auto t0 = std::chrono::steady_clock::now();
   int done = 0;
   while (done < nshot) {
@@ -1190,6 +1191,16 @@ double orc_time_shots(const orc_cfg* cfg, const float* v, const float* r1, const
     for (int i = 0; i < batch; ++i) pthread_join(th[i], nullptr);
     done += batch;
   }
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+double orc_time_rows(const orc_cfg* cfg, const float* v, const float* r1, const float* w,
+                     const float* c, int nc, const int* index, int nvel, const float* wav,
+                     const float* seis, int nthreads, int nsteps_f, int nsteps_b) {
+  TimeJob job{*cfg, v, r1, w, c, wav, seis, index, nc, nvel, nsteps_f, nsteps_b, nthreads};
+  auto t0 = std::chrono::steady_clock::now();
+  time_worker(&job);
   auto t1 = std::chrono::steady_clock::now();
   return std::chrono::duration<double>(t1 - t0).count();
 }

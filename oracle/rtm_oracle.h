@@ -130,6 +130,11 @@ void orc_stack(const float* ups, const float* downs, int nshot, int nI, int inor
 double orc_time_shots(const orc_cfg* cfg, const float* v, const float* r1, const float* w,
                       const float* c, int nc, const int* index, int nvel, const float* wav,
                       const float* seis, int nshot, int nthreads, int nsteps_f, int nsteps_b);
+/* CPU baseline, latency form: one shot whose every step is split by rows over
+ * nthreads pthreads (a barrier per step).  Returns wall seconds. */
+double orc_time_rows(const orc_cfg* cfg, const float* v, const float* r1, const float* w,
+                     const float* c, int nc, const int* index, int nvel, const float* wav,
+                     const float* seis, int nthreads, int nsteps_f, int nsteps_b);
 
 #ifdef __cplusplus
 }

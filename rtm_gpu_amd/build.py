@@ -51,22 +51,37 @@ def _run(cmd, verbose):
         print(f"  done in {time.time() - t:.1f}s", flush=True)
 
 
-def build_hip(verbose=True):
+def _stale(obj, src, deps):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(p) > t for p in [src, *deps])
+
+
+def build_hip(verbose=True, force=True):
+    """Compile the three translation units (in parallel) and link
+    librtm_hip.so.  force=False recompiles only the units whose source or
+    headers changed since their object in lib/obj was written."""
     os.makedirs(LIB, exist_ok=True)
+    obj_dir = os.path.join(LIB, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
     out = os.path.join(LIB, "librtm_hip.so")
     srcs = [os.path.join(CSRC, f) for f in ("rtm_kernels.hip", "rtm_capi.hip", "rtm_cnn.hip")]
-    objs = [os.path.join(LIB, os.path.basename(s) + ".o") for s in srcs]
+    objs = [os.path.join(obj_dir, os.path.basename(s) + ".o") for s in srcs]
+    deps = [os.path.join(CSRC, "rtm_kernels.h"), os.path.join(ROOT, "include", "rtm_hip.h"),
+            os.path.join(ROOT, "include", "rtm_cnn.h")]
+    todo = [(s, o) for s, o in zip(srcs, objs) if force or _stale(o, s, deps)]
     # translation units compile in parallel (hipcc is single-threaded)
     from concurrent.futures import ThreadPoolExecutor
-    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+    with ThreadPoolExecutor(max_workers=max(1, len(todo))) as ex:
         futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", s,
-                                 "-o", o], verbose) for s, o in zip(srcs, objs)]
+                                 "-o", o + ".tmp"], verbose) for s, o in todo]
         for f in futs:
             f.result()
+    for _, o in todo:
+        os.replace(o + ".tmp", o)
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out + ".tmp", *objs], verbose)
     os.replace(out + ".tmp", out)
-    for o in objs:
-        os.remove(o)
     return out
 
 
@@ -92,10 +107,10 @@ def build_host(verbose=True):
     return out
 
 
-def build_all(verbose=True):
-    build_hip(verbose)
+def build_all(verbose=True, force=True):
+    build_hip(verbose, force)
     build_host(verbose)
 
 
 if __name__ == "__main__":
-    build_all(verbose=True)
+    build_all(verbose=True, force="--incremental" not in sys.argv)

@@ -233,7 +233,7 @@ void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
   a.g = c->g;
   a.m = c->m;
   a.r = c->r;
-  a.tiles = c->d_tiles;
+  a.recs = c->d_tiles;
   a.geo = c->d_geo;
   a.nt = c->p.nt;
   a.f64comb = c->p.ilste != 0;
@@ -449,17 +449,14 @@ int adjoint_step(rtm_ctx* c, int k) {
   return adjoint_update(c, k, true);
 }
 
-// Frame tiles of a grid nx columns wide, tagged with the shot.  Shapes must
-// match k_frame: 32 x 64 (top/bottom), 64 x 32 (left/right).
+// Frame records (k_frame) of a grid nx columns wide, tagged with the shot.
 void build_tiles(const Grid& g, int nx, int shot, std::vector<int4>& t) {
-  for (int z0 = 0; z0 < g.n2; z0 += 32)
-    for (int x0 = 0; x0 < nx; x0 += 64) t.push_back(make_int4(z0, x0, 0, shot));
-  for (int z0 = g.nz - g.n2; z0 < g.nz; z0 += 32)
-    for (int x0 = 0; x0 < nx; x0 += 64) t.push_back(make_int4(z0, x0, 1, shot));
-  for (int z0 = g.n2; z0 < g.nz - g.n2; z0 += 64) {
-    for (int x0 = 0; x0 < g.n2; x0 += 32) t.push_back(make_int4(z0, x0, 2, shot));
-    for (int x0 = nx - g.n2; x0 < nx; x0 += 32) t.push_back(make_int4(z0, x0, 3, shot));
-  }
+  const int seg = frame_segment_rows(g.nfd);
+  int n = 0;
+  frame_records(g, nx, shot, seg, nullptr, &n);
+  const size_t at = t.size();
+  t.resize(at + n);
+  frame_records(g, nx, shot, seg, t.data() + at, &n);
 }
 
 int choose_zs(rtm_ctx* c, int nshot);
@@ -1093,6 +1090,49 @@ int rtm_data_axpy(rtm_ctx* c, float a, int shot0, int nshot) {
   int rc = data_range(c, shot0, nshot, &off, &n);
   if (rc != RTM_OK) return rc;
   HIPCHK(c, launch_axpy(c->d_data + off, c->d_rec + off, a, n, c->stream));
+  return RTM_OK;
+}
+
+// ---- streaming copy (bandwidth reference) ----------------------------------
+// A plain float4 copy, four independent 16-byte loads in flight per thread,
+// enough blocks for every CU: the HBM rate a pure streaming kernel reaches on
+// this device, next to which the stencil kernels' rates read.
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ src,
+                                                     float4* __restrict__ dst, long long n4) {
+  const long long stride = (long long)gridDim.x * 1024;
+  for (long long i = (long long)blockIdx.x * 1024 + threadIdx.x; i < n4; i += stride) {
+    float4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long k = i + u * 256;
+      v[u] = k < n4 ? src[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long k = i + u * 256;
+      if (k < n4) dst[k] = v[u];
+    }
+  }
+}
+
+int rtm_stream_copy(const float* src, float* dst, long long n, void* stream) {
+  if (!src || !dst || n < 0 || (n & 3) || ((reinterpret_cast<size_t>(src) |
+                                             reinterpret_cast<size_t>(dst)) & 15)) {
+    t_free_err = "rtm_stream_copy: null, misaligned or n not a multiple of 4";
+    return RTM_E_ARG;
+  }
+  const long long n4 = n / 4;
+  long long blocks = (n4 + 1023) / 1024;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), reinterpret_cast<const float4*>(src),
+                     reinterpret_cast<float4*>(dst), n4);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    t_free_err = std::string("rtm_stream_copy: ") + hipGetErrorString(e);
+    return RTM_E_HIP;
+  }
   return RTM_OK;
 }
 

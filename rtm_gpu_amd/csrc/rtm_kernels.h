@@ -105,12 +105,15 @@ struct FrameArgs {
   const float* u1;
   const float* u0;
   float* u2;
-  const int4* tiles;   // (z0, x0, band, shot)
+  // one record per wave: (first output row, column of lane 0, band | rows << 4,
+  // shot); band 0 top, 1 bottom, 2 left, 3 right (frame_records)
+  const int4* recs;
   const ShotGeo* geo;
   float wav;
   int k, nt;
   int f64comb;
-  int nshot;           // shots of the launch (the fastest-varying block index)
+  int nshot;           // shots of the launch
+  int nrec;            // records of the launch
   const float* data;
   Strips st;
   int save_strips;
@@ -125,6 +128,11 @@ int interior_queue_rows(int epi, int nfd);
 hipError_t launch_interior(int epi, int nfd_inst, IntArgs a, int nshot, hipStream_t s);
 hipError_t launch_frame(int mode, int nfd_inst, FrameArgs a, int nshot, int nrecords,
                         hipStream_t s);
+// Frame records of one shot of grid width nx (k_frame): the top and bottom
+// bands in runs of 64 columns (62 written), the left and right bands in runs
+// of 64 columns by `seg` rows.
+void frame_records(const Grid& g, int nx, int shot, int seg, int4* out, int* count);
+int frame_segment_rows(int nfd);
 hipError_t launch_fill(float* p, long long n, float val, hipStream_t s);
 // Finite-value watch: reset *flag to 0xffffffff; store k in *flag (if still
 // 0xffffffff) when any of the n floats at f (n a multiple of 4, 16-byte

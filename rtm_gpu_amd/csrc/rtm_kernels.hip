@@ -15,6 +15,8 @@
 // flags: -ffp-contract=off, IEEE division, denormals kept), with explicit
 // fused multiply-adds only at the per-tap stencil sum, the tap accumulation,
 // the FP32 final combine and the imaging accumulations.
+#include <algorithm>
+
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -201,7 +203,6 @@ __device__ __forceinline__ void strip_put(const Strips& st, int shot, int k, lon
     p[((long long)shot * st.nslot + (k % st.nslot)) * st.ss + e] = val;
   } else {
     st.stage[((long long)shot * 2 + (k & 1)) * st.ss + e] = val;
-    atomicMax(&st.amax[(long long)shot * st.nslot + (k % st.nslot)], __float_as_uint(fabsf(val)));
   }
 }
 __device__ __forceinline__ float strip_get(const Strips& st, int shot, int k, long long e) {
@@ -518,163 +519,325 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Frame update (hybrid ABC).  A tile of TZ x TX cells of one band:
-//   band 0 top    rows [0, n2)         (32 x 64 tiles)
-//   band 1 bottom rows [nz-n2, nz)     (32 x 64)
-//   band 2 left   rows [n2, nz-n2), cols [0, n2)      (64 x 32)
-//   band 3 right  rows [n2, nz-n2), cols [nx-n2, nx)  (64 x 32)
-// Phase 1 stages u1 on the tile plus a (1+NFD) halo in LDS with the outer
-// mirror of kernel.cu:65-68 applied at load time.  Phase 2 computes the
-// pre-blend two-way value T (with source / data) on the tile plus a 1-cell
-// halo.  Phase 3 computes the one-way value of each ring cell from T, u1, u0
-// (Hybrid1, kernel.cu:116-158) and blends (Hybrid2, kernel.cu:160-183).
+// Frame update (hybrid ABC) as a row march.  The frame is cut into four
+// regions: top rows [0, n2) and bottom rows [nz-n2, nz) over all columns,
+// left columns [0, n2) and right columns [nx-n2, nx) over the rows between.
+// A wave owns 64 consecutive columns of a region (lanes 1..62 write; lanes 0
+// and 63 only supply lateral neighbours) and a run of its rows.  Marching
+// down, iteration t computes on all 64 columns the pre-blend two-way value T
+// of row t -- the value the reference's Hybrid1 reads from the un-blended
+// DFW2: Add / BKAdd on the frame with the source, the data replacement, the
+// Born source or the adjoint residual -- and then the one-way value and the
+// blend of row t-1 (Hybrid1/Hybrid2, kernel.cu:116-183), which read T of rows
+// t-2, t-1 and t: the inner neighbour and its laterals sit one row or one
+// column away.  Vertical taps come from a register queue of u1 rows,
+// horizontal taps from a per-wave LDS row; the reference's outer mirror
+// (kernel.cu:65-68) is applied to the row and column indices of the loads.
+//
+// The row body has no branch (selects; accesses that must not happen get an
+// out-of-range buffer offset), so the memory-counter waits stay exact: the
+// coefficient row of row t+1 is gathered during row t (its offset arrived a
+// row earlier), row operands arrive PD rows ahead and the one-way operands two
+// rows ahead, and each row waits only on loads issued a row or more before.
+// No block-level barrier: the four waves of a block are independent.
+__device__ __forceinline__ float bld1(Rsrc r, unsigned vo, unsigned so) {
+  return __uint_as_float((unsigned)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
+}
+__device__ __forceinline__ unsigned bld1u(Rsrc r, unsigned vo, unsigned so) {
+  return (unsigned)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0);
+}
+__device__ __forceinline__ void bst1(Rsrc r, unsigned vo, unsigned so, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(vo + so), 0, 0);
+}
+
+// Strip element of frame cell (z, x) of a shot nx wide, or -1 (strip_index
+// without branches, in 32 bits: a slot holds fewer than 2^31 elements).
+__device__ __forceinline__ int strip_elem(const Grid& g, int nx, int mnx, int z, int x) {
+  const int a = g.mnz * g.nfd, b = g.nfd * mnx, n2 = g.n2, nfd = g.nfd;
+  const bool zin = z >= n2 && z < g.nz - n2, xin = x >= n2 && x < nx - n2;
+  int e = -1;
+  e = (zin && x >= n2 - nfd && x < n2) ? (z - n2) * nfd + (n2 - 1 - x) : e;
+  e = (zin && x >= nx - n2 && x < nx - n2 + nfd) ? a + (z - n2) * nfd + (x - (nx - n2)) : e;
+  e = (xin && z >= n2 - nfd && z < n2) ? 2 * a + (n2 - 1 - z) * mnx + (x - n2) : e;
+  e = (xin && z >= g.nz - n2 && z < g.nz - n2 + nfd) ? 2 * a + b + (z - (g.nz - n2)) * mnx + (x - n2) : e;
+  return e;
+}
+
 template <int NFD, int MODE>
 __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
-  constexpr int RMAX = (32 + 2 + 2 * NFD) * (64 + 2 + 2 * NFD);
-  constexpr int TMAX = (32 + 2) * (64 + 2);
-  __shared__ float su1[RMAX];
-  __shared__ float sT[TMAX];
+  constexpr int PD = 2;                   // rows of prefetch
+  constexpr int NB = NFD > 2 ? NFD : 2;   // queue rows kept behind the T row
+  // The row loop is unrolled by U, a multiple of 12 that holds the u1 rows
+  // t-NB .. t+NFD+PD; every other per-row buffer is circular with a length
+  // dividing 12 (2, 3 or 6), so no buffer is ever copied, only renamed.
+  constexpr int U = (NB + NFD + 1 + PD + 11) / 12 * 12;
+  constexpr int WN = 64 + 2 * NFD + 1;    // LDS row: halo | 64 columns | halo | dump slot
+  __shared__ float win[4][WN];
   const Grid& g = a.g;
   const Model& m = a.m;
-  // One block per (tile, shot) record; the host lists them tile-major with the
-  // shot fastest, the XCD-friendly order of k_interior.
-  const int4 td = a.tiles[blockIdx.x];
-  const int shot = td.w;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ri = (int)blockIdx.x * 4 + wv;
+  if (ri >= a.nrec) return;
+  const int4 rec = a.recs[ri];
+  const int z0 = rec.x, cx0 = rec.y, band = rec.z & 15, nrows = rec.z >> 4, shot = rec.w;
   const ShotGeo sg = a.geo[shot];
-  const int NX = sg.nx;  // this shot's grid width
-  const int band = td.z;
-  const int TZ = band < 2 ? 32 : 64, TX = band < 2 ? 64 : 32;
-  const int RZ = TZ + 2 + 2 * NFD, RX = TX + 2 + 2 * NFD;
-  const int TRZ = TZ + 2, TRX = TX + 2;
-  const int z0 = td.x, x0 = td.y;
-  const long long so = (long long)shot * g.fstride + g.x0;
-  const float* __restrict__ U1 = a.u1 + so;
-  const float* __restrict__ U0 = a.u0 + so;
-  float* __restrict__ U2 = a.u2 + so;
-  const long long mo = (long long)shot * m.mstride + g.x0;
-  const float* __restrict__ V = m.v + mo;
-  const float* __restrict__ R1 = m.r1 + mo;
-  const float* __restrict__ PM = m.pm + mo;
-  const unsigned* __restrict__ KO = m.ko + mo;
-  const int P = g.pitch;
-  const int2 sp = make_int2(sg.r_u, sg.r_x);
+  const int NX = sg.nx, NZ = g.nz, N2 = g.n2;
+  const int x = cx0 + lane;
+  const int xm = mirror_idx(x, NX);
+  int ccl = 0, ccr = NX;  // output columns of the region
+  if (band == 2) ccr = N2;
+  else if (band == 3) ccl = NX - N2;
+  const bool outcol = lane >= 1 && lane <= 62 && x >= ccl && x < ccr;
+  // A top or bottom run whose written columns all lie in [n2+1, nx-n2-2]
+  // holds only cells of the top (bottom) edges: no corner cell, no side cell
+  // (the ring of (z, x) is z, resp. nz-1-z).  Those waves take a shorter
+  // one-way update; the choice is uniform per wave.
+  const bool pure_tb = band < 2 && cx0 + 1 >= N2 + 1 && cx0 + 62 <= NX - N2 - 2;
+
+  const long long so = (long long)shot * g.fstride;
+  const unsigned fb = (unsigned)g.fstride * 4u;
+  const Rsrc R1 = rsrc(a.u1 + so, fb), R0 = rsrc(a.u0 + so, fb), R2 = rsrc(a.u2 + so, fb);
+  const long long mo = (long long)shot * m.mstride;
+  const Rsrc RP = rsrc(m.pm + mo, fb), RK = rsrc(m.ko + mo, fb);
+  const Rsrc RV = rsrc(m.v + mo, fb), RR = rsrc(m.r1 + mo, fb);
+  const Rsrc RC = rsrc(m.ctab, (unsigned)(m.nrow * m.cs) * 4u);
+  const Rsrc RW = rsrc(m.w, (unsigned)(N2 + 1) * 4u);
+  const unsigned P4 = (unsigned)g.pitch * 4u;
+  auto rowm = [&](int z) -> unsigned { return (unsigned)mirror_idx(z, NZ) * P4; };
+  auto colb = [&](int c) -> unsigned { return (unsigned)(g.x0 + c) * 4u; };
+  const unsigned vcol = colb(xm);
+
+  // halo columns of the LDS row: lanes 0..NFD-1 the left ones, NFD..2NFD-1
+  // the right ones; the other lanes fetch nothing and write the dump slot
+  unsigned vhal = kDrop;
+  int hpos = WN - 1;
+  if (lane < NFD) { vhal = colb(mirror_idx(cx0 - NFD + lane, NX)); hpos = lane; }
+  else if (lane < 2 * NFD) { vhal = colb(mirror_idx(cx0 + 64 + lane - NFD, NX)); hpos = 64 + lane; }
+
+  // receivers on row s_z (kernel.cu:349): this step's data sample of the lane's
+  // trace (receiver field: sample k+1; adjoint: sample k), dropped elsewhere
   Rcv rr = a.r;
   rr.s_l = sg.s_l;
   rr.s_r = sg.s_r;
-
-  for (TileWalk t(RX); t.i < RZ; t.next()) {
-    const int i = t.i, jx = t.j, idx = i * RX + jx;
-    int gz = mirror_idx(z0 - 1 - NFD + i, g.nz);
-    int gx = mirror_idx(x0 - 1 - NFD + jx, NX);
-    su1[idx] = U1[(long)gz * P + gx];
+  int rs = 0;
+  const bool isrcv = (MODE == FR_RCV || MODE == FR_ADJ) && is_receiver(x, rr, &rs);
+  Rsrc RD;
+  if (MODE == FR_RCV || MODE == FR_ADJ) {
+    const int kd = MODE == FR_RCV ? a.k + 1 : a.k;
+    RD = rsrc(a.data + ((long long)shot * a.nt + kd) * a.r.n, (unsigned)a.r.n * 4u);
   }
-  __syncthreads();
+  const unsigned vrcv = isrcv ? (unsigned)rs * 4u : kDrop;
+  // strip slot of this step (FP32) or its FP32 staging (FP16 / BF16)
+  Rsrc RS;
+  if (MODE == FR_FWD) {
+    const long long ss = a.st.ss;
+    const float* sb = a.st.fmt == 0
+        ? (const float*)a.st.base + ((long long)shot * a.st.nslot + a.k % a.st.nslot) * ss
+        : a.st.stage + ((long long)shot * 2 + (a.k & 1)) * ss;
+    RS = rsrc(sb, a.save_strips ? (unsigned)ss * 4u : 0u);
+  }
+  Rsrc RBX, RBA, RBB, RBC;
+  if (MODE == FR_BORN) {
+    RBX = rsrc(m.refl + (long long)shot * m.rstride, fb);
+    RBA = rsrc(a.ua + so, fb); RBB = rsrc(a.ub + so, fb); RBC = rsrc(a.uc + so, fb);
+  }
+  constexpr int NE = MODE == FR_BORN ? 4 : 1;  // extra operands of a T row
 
-  for (TileWalk t(TRX); t.i < TRZ; t.next()) {
-    const int i = t.i, jx = t.j, idx = i * TRX + jx;
-    int gz = z0 - 1 + i, gx = x0 - 1 + jx;
-    float T = 0.f;
-    if (gz >= 0 && gz < g.nz && gx >= 0 && gx < NX) {
-      const int ci = (i + NFD) * RX + (jx + NFD);
-      const long long pc = (long)gz * P + gx;
-      CoefRow<NFD> cr;
-      cr.load(m.ctab, KO[pc]);
-      const float u = su1[ci];
-      float w1 = (float)(cr.c0d() * (double)u);
+  // one-way operands of row zr (velocity, r1, the Q1 velocity, blend weight)
+  auto abc_ops = [&](int zr, float& vp, float& r1v, float& vq, float& wv_) {
+    const int ar = min(min(zr, x), min(NZ - 1 - zr, NX - 1 - x));
+    // row zr as a linear index of the model: row zr / NX, column zr % NX (uniform)
+    const int zdiv = zr / NX, zmod = zr - zdiv * NX;
+    const bool topc = zr == ar, botc = !topc && zr == NZ - 1 - ar;
+    // top: the cell itself; bottom: Dv[(N2-x)*NX+y] (kernel.cu:132); sides:
+    // Dv[(N2-y)*NX+x] (kernel.cu:128,136), a top-band cell
+    int zq = topc ? zr : (botc ? ar : ar + zdiv);
+    int xq = (topc || botc) ? x : zmod;
+    zq = min(max(zq, 0), NZ - 1);
+    xq = min(max(xq, 0), NX - 1);
+    vp = bld1(RV, vcol, rowm(zr));
+    r1v = bld1(RR, vcol, rowm(zr));
+    vq = bld1(RV, colb(xq) + (unsigned)zq * P4, 0u);
+    wv_ = bld1(RW, (unsigned)min(max(N2 - ar, 0), N2) * 4u, 0u);
+  };
+  // extra operands of T row zr: the data sample, or the Born source terms
+  auto t_ops = [&](int zr, float* e) {
+    if (MODE == FR_RCV || MODE == FR_ADJ) e[0] = bld1(RD, zr == a.r.s_z ? vrcv : kDrop, 0u);
+    else if (MODE == FR_BORN) {
+      const unsigned rz = rowm(zr);
+      e[0] = bld1(RBX, vcol, rz); e[1] = bld1(RBA, vcol, rz);
+      e[2] = bld1(RBB, vcol, rz); e[3] = bld1(RBC, vcol, rz);
+    } else {
+      e[0] = 0.f;
+    }
+  };
+
+  // Circular buffers, indexed by the iteration number it (T row t0 + it):
+  //   q[U]     u1 rows (row t0 - NB + i in slot i % U)
+  //   kob[6]   coefficient offsets of T rows it-1 .. it+KD (KD = 4 rows ahead:
+  //            the coefficient row of it+1 is gathered from them one row early)
+  //   pmb, halb [3]: P and halo of T rows it .. it+PD
+  //   u0b[6]   u0 rows it-2 .. it+PD; cr[2] coefficient rows of it, it+1
+  //   abc [3]  one-way operands of rows it-1 .. it+1; ext[2] T-row extras
+  //   T[3]     T rows it-2 .. it
+  const int t0 = z0 - 1;          // T row of the first iteration
+  const int niter = nrows + 2;    // T rows z0-1 .. z0+nrows
+  float q[U];
+#pragma unroll
+  for (int s = 0; s < NB + NFD + PD; ++s) q[s] = bld1(R1, vcol, rowm(t0 - NB + s));
+  constexpr int KD = 4;
+  unsigned kob[6];
+#pragma unroll
+  for (int d = 0; d < KD; ++d) kob[d] = bld1u(RK, vcol, rowm(t0 + d));
+  float pmb[3], halb[3], u0b[6];
+#pragma unroll
+  for (int d = 0; d < PD; ++d) {
+    pmb[d] = bld1(RP, vcol, rowm(t0 + d));
+    halb[d] = bld1(R1, vhal, rowm(t0 + d));
+    u0b[2 + d] = bld1(R0, vcol, rowm(t0 + d));   // slot (it + 2) % 6 holds row t0 + it
+  }
+  u0b[0] = bld1(R0, vcol, rowm(t0 - 2));
+  u0b[1] = bld1(R0, vcol, rowm(t0 - 1));
+  CoefRow<NFD> cr[2];
+  cr[0].load(RC, kob[0]);
+  float ext[2][NE];
+  t_ops(t0, ext[0]);
+  float T[3] = {0.f, 0.f, 0.f};
+  float vpb[3] = {0.f, 0.f, 0.f}, r1b[3] = {0.f, 0.f, 0.f}, vqb[3] = {0.f, 0.f, 0.f},
+        wb[3] = {0.f, 0.f, 0.f};   // slot (it + 1) % 3 holds row t0 + it
+
+  for (int base = 0; base < niter; base += U) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int it = base + j;   // base is a multiple of U, so it % L == j % L for L | 12
+      if (it >= niter) break;
+      const int tt = t0 + it;
+      // (1) coefficient row of T row tt+1 (its offset arrived a row ago)
+      cr[(j + 1) % 2].load(RC, kob[(j + 1) % 6]);
+      // (2) operands of later rows
+      kob[(j + KD) % 6] = bld1u(RK, vcol, rowm(tt + KD));
+      q[(j + NB + NFD + PD) % U] = bld1(R1, vcol, rowm(tt + NFD + PD));
+      pmb[(j + PD) % 3] = bld1(RP, vcol, rowm(tt + PD));
+      halb[(j + PD) % 3] = bld1(R1, vhal, rowm(tt + PD));
+      u0b[(j + 2 + PD) % 6] = bld1(R0, vcol, rowm(tt + PD));
+      abc_ops(tt + 1, vpb[(j + 2) % 3], r1b[(j + 2) % 3], vqb[(j + 2) % 3], wb[(j + 2) % 3]);
+      t_ops(tt + 1, ext[(j + 1) % 2]);
+      __builtin_amdgcn_sched_barrier(0);
+      // (3) horizontal neighbours of row tt through LDS
+      const float u = q[(j + NB) % U];
+      win[wv][NFD + lane] = u;
+      win[wv][hpos] = halb[j % 3];
+      wave_lds_sync();
+      float hl[NFD], hr[NFD];
 #pragma unroll
       for (int l = 1; l <= NFD; ++l) {
-        float s = su1[ci - l * RX] + su1[ci + l * RX];
-        s = __builtin_fmaf(s, m.hzx2_1, su1[ci - l]);
-        s = s + su1[ci + l];
-        w1 = __builtin_fmaf(cr.c(l), s, w1);
+        hl[l - 1] = win[wv][NFD + lane - l];
+        hr[l - 1] = win[wv][NFD + lane + l];
       }
-      const float u0 = U0[pc];
+      wave_lds_sync();
+      // (4) pre-blend two-way value T of (tt, x) (Add / BKAdd, kernel.cu:46-80, 339-380)
+      const CoefRow<NFD>& crc = cr[j % 2];
+      float w1 = (float)(crc.c0d() * (double)u);
+#pragma unroll
+      for (int l = 1; l <= NFD; ++l) {
+        float s = q[(j + NB - l + U) % U] + q[(j + NB + l) % U];
+        s = __builtin_fmaf(s, m.hzx2_1, hl[l - 1]);
+        s = s + hr[l - 1];
+        w1 = __builtin_fmaf(crc.c(l), s, w1);
+      }
+      const float u0 = u0b[(j + 2) % 6], pmv = pmb[j % 3];
+      const float* ex = ext[j % 2];
+      float Tn;
       if (MODE == FR_FWD) {
-        T = combine_p(u, u0, PM[pc], w1, a.f64comb != 0);
-        if (gz == sp.x && gx == sp.y) T = T + a.wav;
+        Tn = combine_p(u, u0, pmv, w1, a.f64comb != 0);
+        Tn = (tt == sg.r_u && x == sg.r_x) ? Tn + a.wav : Tn;
       } else if (MODE == FR_BORN) {  // scattered field: m * D2u on the interior, no source
-        T = combine_p(u, u0, PM[pc], w1, a.f64comb != 0);
-        if (gz >= g.n2 && gz < g.nz - g.n2 && gx >= g.n2 && gx < NX - g.n2) {
-          const float d2u = (a.ua[so + pc] - 2.0f * a.ub[so + pc]) + a.uc[so + pc];
-          T = __builtin_fmaf(m.refl[(long long)shot * m.rstride + g.x0 + pc], d2u, T);
-        }
+        Tn = combine_p(u, u0, pmv, w1, a.f64comb != 0);
+        const float d2u = (ex[1] - 2.0f * ex[2]) + ex[3];
+        const bool inner = tt >= N2 && tt < NZ - N2 && x >= N2 && x < NX - N2;
+        Tn = inner ? __builtin_fmaf(ex[0], d2u, Tn) : Tn;
       } else if (MODE == FR_ADJ) {   // adjoint field: P * r(k) added at the receivers
-        T = combine_p(u, u0, PM[pc], w1, false);
-        int s;
-        if (gz == a.r.s_z && is_receiver(gx, rr, &s))
-          T = T + PM[pc] * a.data[((long long)shot * a.nt + a.k) * a.r.n + s];
-      } else {
-        int s;
-        bool inj = false;
-        if (gz == a.r.s_z && is_receiver(gx, rr, &s)) {
-          float d = a.data[((long long)shot * a.nt + (a.k + 1)) * a.r.n + s];
-          if (d != 0.0f) { T = d; inj = true; }
+        Tn = combine_p(u, u0, pmv, w1, false);
+        Tn = (isrcv && tt == a.r.s_z) ? Tn + pmv * ex[0] : Tn;
+      } else {                       // receiver field: data replace where non-zero
+        Tn = combine_p(u, u0, pmv, w1, false);
+        Tn = (isrcv && tt == a.r.s_z && ex[0] != 0.0f) ? ex[0] : Tn;
+      }
+      T[j % 3] = Tn;
+      // (5) one-way value and blend of row z = tt - 1 (written from the third row on)
+      if (pure_tb) {  // top edge (inner row z+1) or bottom edge (inner row z-1) (kernel.cu:122-125,130-133)
+        const int z = tt - 1;
+        const bool tp = band == 0;
+        const float T0 = T[(j + 2) % 3];
+        const float Tin = tp ? T[j % 3] : T[(j + 1) % 3];
+        const float U1z = q[(j + NB - 1) % U];
+        const float U1i = tp ? q[(j + NB) % U] : q[(j + NB - 2) % U];
+        const float U0z = u0b[(j + 1) % 6];
+        const float U0i = tp ? u0b[(j + 2) % 6] : u0b[j % 6];
+        const float Til = __shfl_up(Tin, 1), Tir = __shfl_down(Tin, 1);
+        const float U0l = __shfl_up(U0z, 1), U0r = __shfl_down(U0z, 1);
+        const int ab = j % 3;
+        const float vp = vpb[ab], vq = vqb[ab], wgt = wb[ab];
+        const float tv = m.taoh * vp;
+        const float t1 = tv * ((Tin - U0i) + U0z);
+        const float t2 = ((((-2.0f * U1z) + U0z) + Tin) - 2.0f * U1i) + U0i;
+        const float lap = ((((Tir - 2.0f * Tin) + Til) + U0r) - 2.0f * U0z) + U0l;
+        const float t3 = ((m.taoh2 * vq) * vq) * lap;
+        const float O = (1.0f / (tv + 1.0f)) * ((t1 - t2) + t3);
+        const float val = (1.0f - wgt) * T0 + wgt * O;
+        const bool out = outcol && it >= 2 && z < z0 + nrows;
+        bst1(R2, out ? vcol : kDrop, rowm(z), val);
+        if (MODE == FR_FWD) {  // Hybrid3 (kernel.cu:184-208)
+          const int e = strip_elem(g, NX, sg.mnx, z, x);
+          bst1(RS, (out && e >= 0) ? (unsigned)e * 4u : kDrop, 0u, val);
         }
-        if (!inj) T = combine_p(u, u0, PM[pc], w1, false);
-      }
-    }
-    sT[idx] = T;
-  }
-  __syncthreads();
-
-  for (TileWalk t(TX); t.i < TZ; t.next()) {
-    const int i = t.i, jx = t.j;
-    const int z = z0 + i, x = x0 + jx;
-    bool in;
-    if (band == 0) in = z < g.n2 && x < NX;
-    else if (band == 1) in = z >= g.nz - g.n2 && z < g.nz && x < NX;
-    else if (band == 2) in = z >= g.n2 && z < g.nz - g.n2 && x < g.n2;
-    else in = z >= g.n2 && z < g.nz - g.n2 && x >= NX - g.n2 && x < NX;
-    if (!in) continue;
-    const int ar = min(min(z, x), min(g.nz - 1 - z, NX - 1 - x));  // ring
-    const int br = g.nz - 1 - ar, crr = NX - 1 - ar;
-    const long long pg = (long)z * P + x;
-    auto Tat = [&](int dz, int dx) { return sT[(i + 1 + dz) * TRX + (jx + 1 + dx)]; };
-    auto U1at = [&](int dz, int dx) { return su1[(i + 1 + NFD + dz) * RX + (jx + 1 + NFD + dx)]; };
-    const float Tp = Tat(0, 0);
-    const bool zc = (z - ar <= 1) || (br - z <= 1);
-    const bool xc = (x - ar <= 1) || (crr - x <= 1);
-    float O;
-    if (zc && xc) {  // corner cells (kernel.cu:138-155)
-      const int dzc = (z - ar <= 1) ? 1 : -1, dxc = (x - ar <= 1) ? 1 : -1;
-      const float rr = R1[pg];
-      O = (1.0f / (2.0f * rr + 1.0f)) * (U1at(0, 0) + rr * (Tat(0, dxc) + Tat(dzc, 0)));
-    } else {
-      int dzi, dxi;        // toward the inner neighbour
-      int lz, lx;          // lateral (+) direction
-      float vq;            // velocity of the taoh2*v^2 term (kernel.cu:124,128,132,136)
-      const float vp = V[pg];
-      if (z == ar) { dzi = 1; dxi = 0; lz = 0; lx = 1; vq = vp; }
-      else if (z == br) { dzi = -1; dxi = 0; lz = 0; lx = 1; vq = V[(long)ar * P + x]; }
-      else {
-        dzi = 0; lz = 1; lx = 0;
-        dxi = (x == ar) ? 1 : -1;
-        // Dv[(N2-y)*NX+x] as a linear index: row ar + z/nx, column z%nx
-        int zq = ar, xq = z;
-        if (xq >= NX) { zq += xq / NX; xq -= (xq / NX) * NX; }
-        vq = V[(long)zq * P + xq];
-      }
-      const long long pi = pg + (long)dzi * P + dxi;
-      const long long lat = (long)lz * P + lx;
-      const float U0p = U0[pg], U0i = U0[pi];
-      const float Ti = Tat(dzi, dxi);
-      const float Tip = Tat(dzi + lz, dxi + lx), Tim = Tat(dzi - lz, dxi - lx);
-      const float U1p = U1at(0, 0), U1i = U1at(dzi, dxi);
-      const float U0pp = U0[pg + lat], U0pm = U0[pg - lat];
-      const float tv = m.taoh * vp;
-      const float t1 = tv * ((Ti - U0i) + U0p);
-      const float t2 = ((((-2.0f * U1p) + U0p) + Ti) - 2.0f * U1i) + U0i;
-      const float lap = ((((Tip - 2.0f * Ti) + Tim) + U0pp) - 2.0f * U0p) + U0pm;
-      const float t3 = ((m.taoh2 * vq) * vq) * lap;
-      O = (1.0f / (tv + 1.0f)) * ((t1 - t2) + t3);
-    }
-    const float wgt = m.w[g.n2 - ar];
-    const float val = (1.0f - wgt) * Tp + wgt * O;
-    U2[pg] = val;
-    if (MODE == FR_FWD) {
-      if (a.save_strips) {
-        long long e = strip_index(g, NX, sg.mnx, z, x);
-        if (e >= 0) strip_put(a.st, shot, a.k, e, val);
+      } else {
+        const int z = tt - 1;
+        const float Tm = T[(j + 1) % 3], T0 = T[(j + 2) % 3], Tp = T[j % 3];
+        const float U1m = q[(j + NB - 2) % U], U1z = q[(j + NB - 1) % U], U1p = q[(j + NB) % U];
+        const float U0m = u0b[j % 6], U0z = u0b[(j + 1) % 6], U0p = u0b[(j + 2) % 6];
+        const float T0l = __shfl_up(T0, 1), T0r = __shfl_down(T0, 1);
+        const float Tml = __shfl_up(Tm, 1), Tmr = __shfl_down(Tm, 1);
+        const float Tpl = __shfl_up(Tp, 1), Tpr = __shfl_down(Tp, 1);
+        const float U0l = __shfl_up(U0z, 1), U0r = __shfl_down(U0z, 1);
+        const float U1l = __shfl_up(U1z, 1), U1r = __shfl_down(U1z, 1);
+        const int ar = min(min(z, x), min(NZ - 1 - z, NX - 1 - x));  // ring
+        const int br = NZ - 1 - ar, crr = NX - 1 - ar;
+        const bool zc = (z - ar <= 1) || (br - z <= 1);
+        const bool xc = (x - ar <= 1) || (crr - x <= 1);
+        const bool corner = zc && xc;
+        const bool top = z == ar, bot = !top && z == br, side = !top && !bot;
+        const bool inr = x == ar;  // side cell with its inner neighbour to the right
+        const int ab = j % 3;      // one-way operands of row z
+        const float vp = vpb[ab], r1v = r1b[ab], vq = vqb[ab], wgt = wb[ab];
+        // corner cells (kernel.cu:138-155)
+        const float Tx = (x - ar <= 1) ? T0r : T0l;
+        const float Tz = (z - ar <= 1) ? Tp : Tm;
+        const float num_c = U1z + r1v * (Tx + Tz);
+        const float den_c = 2.0f * r1v + 1.0f;
+        // one-way update of the edges (kernel.cu:122-137)
+        const float Ti = top ? Tp : (bot ? Tm : (inr ? T0r : T0l));
+        const float Tip = top ? Tpr : (bot ? Tmr : (inr ? Tpr : Tpl));
+        const float Tim = top ? Tpl : (bot ? Tml : (inr ? Tmr : Tml));
+        const float U0i = top ? U0p : (bot ? U0m : (inr ? U0r : U0l));
+        const float U0pp = side ? U0p : U0r, U0pm = side ? U0m : U0l;
+        const float U1i = top ? U1p : (bot ? U1m : (inr ? U1r : U1l));
+        const float tv = m.taoh * vp;
+        const float t1 = tv * ((Ti - U0i) + U0z);
+        const float t2 = ((((-2.0f * U1z) + U0z) + Ti) - 2.0f * U1i) + U0i;
+        const float lap = ((((Tip - 2.0f * Ti) + Tim) + U0pp) - 2.0f * U0z) + U0pm;
+        const float t3 = ((m.taoh2 * vq) * vq) * lap;
+        const float num_s = (t1 - t2) + t3;
+        const float den_s = tv + 1.0f;
+        const float O = (1.0f / (corner ? den_c : den_s)) * (corner ? num_c : num_s);
+        const float val = (1.0f - wgt) * T0 + wgt * O;
+        const bool out = outcol && it >= 2 && z < z0 + nrows;
+        bst1(R2, out ? vcol : kDrop, rowm(z), val);
+        if (MODE == FR_FWD) {  // Hybrid3 (kernel.cu:184-208)
+          const int e = strip_elem(g, NX, sg.mnx, z, x);
+          bst1(RS, (out && e >= 0) ? (unsigned)e * 4u : kDrop, 0u, val);
+        }
       }
     }
   }
@@ -732,6 +895,25 @@ __global__ void k_strip_save(Grid g, const float* f, Strips st, const ShotGeo* g
     strip_cell(g, sg.nx, sg.mnx, e, &z, &x);
     strip_put(st, shot, k, e, F[(long)z * g.pitch + x]);
   }
+}
+
+// Maximum |value| of the staged fp32 strips of slot k: one atomic per block
+// on the bit patterns (non-negative floats order as their bits).
+__global__ __launch_bounds__(256) void k_strip_amax(Grid g, Strips st, const ShotGeo* geo, int k) {
+  __shared__ unsigned sh[4];
+  const int shot = blockIdx.y;
+  const long long ss = strip_size(g, geo[shot].mnx);
+  const float* s = st.stage + ((long long)shot * 2 + (k & 1)) * st.ss;
+  unsigned mx = 0u;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ss;
+       e += (long long)gridDim.x * blockDim.x)
+    mx = max(mx, __float_as_uint(fabsf(s[e])));
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, o));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicMax(&st.amax[(long long)shot * st.nslot + (k % st.nslot)],
+              max(max(sh[0], sh[1]), max(sh[2], sh[3])));
 }
 
 // Quantise the staged fp32 strips of slot k with the slot's power-of-two scale.
@@ -1097,9 +1279,30 @@ hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s
   return hipGetLastError();
 }
 
+int frame_segment_rows(int nfd) {
+  const int nb = nfd > 2 ? nfd : 2;
+  return 2 * (nb + nfd + 1 + 2) - 2;  // two register-queue cycles per side run
+}
+void frame_records(const Grid& g, int nx, int shot, int seg, int4* out, int* count) {
+  int n = 0;
+  auto put = [&](int z0, int cx0, int band, int rows) {
+    if (out) out[n] = make_int4(z0, cx0, band | (rows << 4), shot);
+    ++n;
+  };
+  for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(0, cx0, 0, g.n2);
+  for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(g.nz - g.n2, cx0, 1, g.n2);
+  for (int z0 = g.n2; z0 < g.nz - g.n2; z0 += seg) {
+    const int rows = std::min(seg, g.nz - g.n2 - z0);
+    for (int cx0 = -1; cx0 + 1 < g.n2; cx0 += 62) put(z0, cx0, 2, rows);
+    for (int cx0 = nx - g.n2 - 1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, 3, rows);
+  }
+  *count = n;
+}
+
 hipError_t launch_frame(int mode, int nfd, FrameArgs a, int nshot, int nrec, hipStream_t s) {
   a.nshot = nshot;
-  dim3 grid(nrec), block(256);  // one block per (tile, shot) record
+  a.nrec = nrec;
+  dim3 grid((nrec + 3) / 4), block(256);  // one wave per (region run, shot) record
 #define CASE(N)                                                                          \
   case N:                                                                                \
     if (mode == FR_FWD) hipLaunchKernelGGL((k_frame<N, FR_FWD>), grid, block, 0, s, a);  \
@@ -1150,6 +1353,8 @@ hipError_t launch_strip_save(const Grid& g, const float* f, Strips st, const Sho
 }
 hipError_t launch_strip_finish(const Grid& g, Strips st, const ShotGeo* geo, int k, int nshot,
                                hipStream_t s) {
+  hipLaunchKernelGGL(k_strip_amax, dim3(blocks_for(strip_size_host(g), 256, 64), nshot), dim3(256),
+                     0, s, g, st, geo, k);
   hipLaunchKernelGGL(k_strip_finish, dim3(blocks_for(strip_size_host(g)), nshot), dim3(256), 0, s,
                      g, st, geo, k);
   return hipGetLastError();

@@ -93,6 +93,8 @@ def lib():
         L.rtm_data_axpy.argtypes = [vp, C.c_float, C.c_int, C.c_int]
         L.rtm_data_dot_noalloc.argtypes = [vp, vp, C.c_longlong, C.c_longlong, C.c_int, vp, vp]
         L.rtm_data_dot_noalloc.restype = C.c_int
+        L.rtm_stream_copy.argtypes = [vp, vp, C.c_longlong, vp]
+        L.rtm_stream_copy.restype = C.c_int
         L.rtm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",

@@ -36,7 +36,10 @@ def build_case(name="small", mod_nz=40, mod_nx=60, n2=10, nfdmax=4, nfdmin=2, nt
     if n is None:
         n = (mod_nx - s_l1) // ds + 1
     d = ob.derive(mod_nz, mod_nx, n2, nt, tao, tao, h, hz, f0, s_l1, s_z1, n, ds, src_x1)
-    if kind == "faulted":  # the seeded reverse-VSP model (8 layers, 3 faults, a body)
+    if kind == "layered5":  # the seeded single-shot model: 5 flat layers, 1500-4500 m/s
+        from rtm_gpu_amd import synth
+        raw = synth.layered_5(seed=seed, mod_nz=mod_nz, mod_nx=mod_nx)
+    elif kind == "faulted":  # the seeded reverse-VSP model (8 layers, 3 faults, a body)
         from rtm_gpu_amd import synth
         raw = synth.faulted_rvsp(seed=seed, mod_nz=mod_nz, mod_nx=mod_nx, h=h)
         if smooth:  # Gaussian-smoothed migration velocity: thousands of velocity bins

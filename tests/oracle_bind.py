@@ -88,6 +88,9 @@ def lib():
         L.orc_time_shots.argtypes = [C.POINTER(OrcCfg), fp, fp, fp, fp, C.c_int, C.c_void_p,
                                      C.c_int, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_time_shots.restype = C.c_double
+        L.orc_time_rows.argtypes = [C.POINTER(OrcCfg), fp, fp, fp, fp, C.c_int, C.c_void_p,
+                                    C.c_int, fp, fp, C.c_int, C.c_int, C.c_int]
+        L.orc_time_rows.restype = C.c_double
         _lib = L
     return _lib
 
