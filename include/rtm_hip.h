@@ -191,10 +191,11 @@ int rtm_stack_finish(rtm_ctx* ctx, int nrec, int inorm, float* img, float* illum
  * the migration; the adjoint field is carried as mu = P*lambda with
  * P = v^2*tao2*h2 per cell, so it takes the plain update and ABC with
  * P*r(k) added at the receivers, and the gradient is
- * (sum_k ((u(k+2) - 2u(k+1)) + u(k)) * mu(k+2)) / P.  The pair passes the
- * dot-product test to the accuracy of the ABC when the model has one
- * coefficient row (ilste = 1); per-cell LS rows make the spatial operator
- * non-symmetric. */
+ * (sum_k ((u(k+2) - 2u(k+1)) + u(k)) * mu(k+2)) / P.  With per-cell LS rows
+ * (ilste = 0) the adjoint field takes the transposed stencil on the interior
+ * (each tap with the coefficient of the neighbour's own row); one Taylor row is
+ * symmetric.  The pair passes the dot-product test to the accuracy of the
+ * hybrid ABC, which is not self-adjoint. */
 
 /* Reflectivity m on the stacked image grid [mnz][stack_nx]; each shot reads
  * its window at x_img.  Needs the shots to be set. */
@@ -251,15 +252,20 @@ int rtm_data_dot_noalloc(const float* x, const float* y, long long n, long long 
 
 /* Autotuner (not in the reference): for each interior kernel of the
  * migration (forward, source reconstruction, receiver step + imaging) times
- * nsteps launches of that kernel on the current batch for each candidate
- * height of its row segments (the kernels round it to a multiple of their
- * register-queue length) and keeps the fastest per kernel for this context
- * (Born modelling and the adjoint take the receiver step's); zseg[3], when
- * not NULL, receives the three choices.  Needs model, wavelet and shots;
- * leaves the passes to be begun anew.  Rows per wave trade the vertical
- * halo re-read (2*NFD rows per segment) against the number of waves in
- * flight. */
+ * that kernel inside nsteps forward + backward step pairs of the current batch
+ * (the other kernels interleaved, the buffers rotating as in a migration) for
+ * each candidate height of its row segments (rounded by the kernel to a
+ * multiple of its register-queue length), one kernel at a time, and keeps the
+ * fastest per kernel for this context (Born modelling and the adjoint take the
+ * receiver step's); zseg[3], when not NULL, receives the three choices.  Needs
+ * model, wavelet and shots; leaves the passes to be begun anew and resets the
+ * rtm_kernel_time counters.  Rows per wave trade the vertical halo re-read
+ * (2*NFD rows per segment) against the number of waves in flight. */
 int rtm_autotune(rtm_ctx* ctx, int nsteps, int* zseg);
+/* The autotuner's three choices set by hand: rows per wave of the forward,
+ * the source-reconstruction and the receiver-step kernels (Born and the
+ * adjoint take the receiver step's). */
+int rtm_set_zseg(rtm_ctx* ctx, const int* zseg);
 
 /* Kernel timing with HIP events on the context stream: mask is a bit set of
  * kernel kinds (1 << RTM_K_*), 0 = off, -1 = all.  Resets the counters. */

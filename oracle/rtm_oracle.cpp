@@ -196,6 +196,46 @@ struct Engine {
     }
     return w1;
   }
+  // Tap l of the coefficient row of cell (x, y) (row-major, x = row z), 0 past
+  // the row's operator length; and the longest operator of the table.
+  inline float tap_of(int x, int y, int l) const {
+    int kb_ = bin_of(v[(size_t)x * NX + y]);
+    int M = index[kb_ + 1] - index[kb_] - 1;
+    return l <= M ? c[index[kb_] + l] : 0.0f;
+  }
+  int lmax = 0;  // longest operator of the table (set with the model)
+  void set_lmax() {
+    lmax = 0;
+    for (int k = 0; k < nvel && !index.empty(); ++k) lmax = std::max(lmax, index[k + 1] - index[k] - 1);
+  }
+  // Transposed stencil sum at interior cell (x, y) for per-cell (LS) rows: the
+  // spatial operator (L u)_i = (1+rho) c0(i) u_i + sum_l c_l(i) S_l(u)_i, with
+  // S_l(u)_i = rho (u_{z-l} + u_{z+l}) + u_{x-l} + u_{x+l}, has the transpose
+  // (L^T u)_i = (1+rho) c0(i) u_i + sum_l [rho (c_l(z-l) u_{z-l} + c_l(z+l) u_{z+l})
+  //             + c_l(x-l) u_{x-l} + c_l(x+l) u_{x+l}]:
+  // each tap takes the coefficient of the neighbour's own row.  Taps run to the
+  // longest operator of the table (zero past a row's length).  Interior cells
+  // only (the taps stay in the grid since nfdmax <= n2).
+  inline R w1T_at(const R* U1, int x, int y) const {
+    int kb_ = bin_of(v[(size_t)x * NX + y]);
+    const float c0 = c[index[kb_]];
+    R u = U1[(size_t)x * NX + y];
+    R w1;
+    if (std::is_same<R, float>::value)
+      w1 = (R)((1.0 + (double)g.hzx2_1) * (double)c0 * (double)u);
+    else
+      w1 = (R)((1.0 + (double)g.hzx2_1) * (double)c0) * u;
+    const R hz = (R)g.hzx2_1;
+    for (int l = 1; l <= lmax; ++l) {
+      const R cu = (R)tap_of(x - l, y, l), cd = (R)tap_of(x + l, y, l);
+      const R cl = (R)tap_of(x, y - l, l), cr = (R)tap_of(x, y + l, l);
+      R vs = fmaR<R>(cu, U1[(size_t)(x - l) * NX + y], cd * U1[(size_t)(x + l) * NX + y]);
+      R s = fmaR<R>(vs, hz, cl * U1[(size_t)x * NX + y - l]);
+      s = fmaR<R>(cr, U1[(size_t)x * NX + y + l], s);
+      w1 = w1 + s;
+    }
+    return w1;
+  }
   // Final combine.  f64comb selects the double form "2.0*u1-u0+P".
   inline R combine(R u1, R u0, float vv, R w1, bool f64comb) const {
     if (std::is_same<R, float>::value) {
@@ -536,11 +576,11 @@ struct Engine {
   }
 
   // ---- adjoint of Born modelling.  The update is A = 2I + P*L with
-  // P = v^2*tao2*h2 per cell, so A^T = P^-1 A P: the adjoint recursion
+  // P = v^2*tao2*h2 per cell, so A^T = 2I + L^T P: the adjoint recursion
   // lambda(k) = A^T lambda(k+1) - lambda(k+2) + R^T r(k) is carried as
-  // mu = P*lambda, which takes the same update A and ABC with P*r(k) added at
-  // the receivers (exact when L is symmetric, i.e. one coefficient row for all
-  // cells).  The background is reconstructed backward from the strips as in
+  // mu = P*lambda, which takes the update 2I + P L^T and the ABC with P*r(k)
+  // added at the receivers.  L^T is the transposed stencil (w1T_at) on the
+  // interior for per-cell rows, and L itself for one (symmetric) Taylor row.  The background is reconstructed backward from the strips as in
   // the migration.  G accumulates ((u(k+2) - 2u(k+1)) + u(k)) * mu(k+2) on
   // the interior; the gradient is G / P (adjoint_image).  mu(nt-1) and
   // mu(nt-2) come from two updates before the loop; their image terms pair
@@ -586,10 +626,15 @@ struct Engine {
     const R* Q1 = E.Q[E.q1].data();
     const R* Q0 = E.Q[E.q0].data();
     R* Q2 = E.Q[E.q2].data();
+    // per-cell LS rows: the interior takes the transposed stencil (exact
+    // adjoint of the interior update); one Taylor row is symmetric already.
+    // The frame keeps the forward update and ABC.
+    const bool tr = E.g.ilste == 0;
     for (int x = b; x < e; ++x)
       for (int y = 0; y < E.NX; ++y) {
         size_t p = (size_t)x * E.NX + y;
-        R w1 = E.w1_at(Q1, x, y);
+        const bool inner = x >= E.N2 && x < E.NZ - E.N2 && y >= E.N2 && y < E.NX - E.N2;
+        R w1 = (tr && inner) ? E.w1T_at(Q1, x, y) : E.w1_at(Q1, x, y);
         Q2[p] = E.combine(Q1[p], Q0[p], E.v[p], w1, false);
       }
   }
@@ -1070,6 +1115,7 @@ static void set_model_t(Engine<R>* e, const float* v, const float* r1, const flo
   e->c.assign(c, c + nc);
   e->nvel = nvel;
   if (index) e->index.assign(index, index + nvel + 1);
+  e->set_lmax();
   e->wav.assign(wav, wav + e->NT + 1);
 }
 }  // extern "C++"

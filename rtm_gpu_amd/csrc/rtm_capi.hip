@@ -78,6 +78,9 @@ struct rtm_ctx {
   size_t refl_cap = 0;
   bool refl_ok = false;
   float* d_grad = nullptr;      // [mnz][gw]
+  float* d_tp = nullptr;        // neighbour-tap planes of the transposed stencil
+  size_t tp_cap = 0;
+  bool tp_ok = false;           // planes match the current model windows
   float* d_stage = nullptr;     // one shot's trace-major gather [n][nt] (host transfers)
   double* d_dot = nullptr;      // FP64 dot-product scratch
   // finite-value watch: every check_every steps the newest fields are checked
@@ -241,6 +244,7 @@ void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
   a.st = c->st;
   a.save_strips = 0;
   a.ua = a.ub = a.uc = nullptr;
+  a.adjt = 0;
 }
 
 // ---- finite-value watch -------------------------------------------------------
@@ -407,14 +411,18 @@ int adjoint_update(rtm_ctx* c, int k, bool image) {
     a.image = image ? 1 : 0;
     a.wav = 0.f; a.k = k;
     Timed t(c, RTM_K_ADJ_INTERIOR);
+    // per-cell (LS) rows: the transposed stencil, so the pair is exactly adjoint
+    // on the interior; one Taylor row is symmetric
+    const int epi = c->p.ilste == 0 ? EPI_ADJT : EPI_ADJ;
     a.zs = zs_for(c, EPI_ADJ);
-    HIPCHK(c, launch_interior(EPI_ADJ, c->nfd, a, c->nshot, c->stream));
+    HIPCHK(c, launch_interior(epi, c->nfd, a, c->nshot, c->stream));
   }
   {
     FrameArgs a;
     fill_frame_args(c, a);
     a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
     a.wav = 0.f; a.k = k;
+    a.adjt = c->p.ilste == 0;
     Timed t(c, RTM_K_ADJ_FRAME);
     HIPCHK(c, launch_frame(FR_ADJ, c->nfd, a, c->nshot, c->ntiles, c->stream));
   }
@@ -757,6 +765,7 @@ int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, c
   m.hzx2_1 = c->p.hzx2_1; m.tao2 = c->p.tao2; m.h2 = c->p.h2;
   m.taoh = c->p.taoh; m.taoh2 = c->p.taoh2;
   c->nfd = nfd;
+  c->tp_ok = false;
   drop_graph(c);
   c->model_ok = true;
   return RTM_OK;
@@ -832,6 +841,7 @@ int rtm_set_shot_model(rtm_ctx* c, int shot, const float* v, const float* r1) {
   m.mstride = g.fstride;
   m.v = c->d_mv; m.r1 = c->d_mr1; m.pm = c->d_mpm; m.ko = c->d_mko;
   c->shot_model[shot] = 1;
+  c->tp_ok = false;
   drop_graph(c);
   return RTM_OK;
 }
@@ -1005,6 +1015,27 @@ int rtm_adjoint_begin(rtm_ctx* c) {
   if (c->kf != c->p.nt) return fail(c, RTM_E_STATE, "forward pass incomplete (k=%d of %d)", c->kf, c->p.nt);
   if (!c->save_strips) return fail(c, RTM_E_STATE, "forward pass ran without strips");
   if (!c->data_ok) return fail(c, RTM_E_STATE, "rtm_set_gathers first (residuals)");
+  if (c->p.ilste == 0 && !c->tp_ok) {  // neighbour-tap planes of the current model windows
+    const Grid& g = c->g;
+    const int nwin = c->m.mstride ? c->nshot : 1;
+    const size_t per = (size_t)4 * c->nfd * g.fstride;
+    if (per * sizeof(float) >= (1ull << 31))
+      return fail(c, RTM_E_ARG, "grid too large for the transposed adjoint stencil");
+    if (c->tp_cap < per * nwin) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      dfree_t(c, &c->d_tp, c->tp_cap);
+      c->tp_cap = 0;
+      int rc = dalloc_t(c, &c->d_tp, per * nwin);
+      if (rc != RTM_OK) return rc;
+      c->tp_cap = per * nwin;
+    }
+    for (int w = 0; w < nwin; ++w)
+      HIPCHK(c, launch_tap_planes(g, c->m.ko + (size_t)w * c->m.mstride, c->m.ctab, c->nfd,
+                                  c->d_tp + (size_t)w * per, c->stream));
+    c->m.tp = c->d_tp;
+    c->m.tpstride = c->m.mstride ? (long long)per : 0;
+    c->tp_ok = true;
+  }
   std::swap(c->f0, c->f1);  // f1 = u(nt-2), f0 = u(nt-1), as for the migration
   int wrc = watch_reset(c);
   if (wrc != RTM_OK) return wrc;
@@ -1261,63 +1292,94 @@ int rtm_set_graph(rtm_ctx* c, int on) {
   return RTM_OK;
 }
 
+int rtm_set_zseg(rtm_ctx* c, const int* zseg) {
+  if (!c || !zseg) return RTM_E_ARG;
+  for (int i = 0; i < 3; ++i)
+    if (zseg[i] < 1 || zseg[i] > 4096) return fail(c, RTM_E_ARG, "rows per wave %d", zseg[i]);
+  const int fwd = c->p.ilste ? EPI_FWD64 : EPI_FWD;
+  c->zs_epi[fwd] = zseg[0];
+  c->zs_epi[EPI_SRC] = c->zs_epi[EPI_SRCI] = zseg[1];
+  c->zs_epi[EPI_RCV] = c->zs_epi[EPI_RCVI] = zseg[2];
+  c->zs_epi[EPI_BORN] = c->zs_epi[EPI_BORN64] = c->zs_epi[EPI_ADJ] = zseg[2];
+  c->zs = zseg[0];
+  drop_graph(c);
+  return RTM_OK;
+}
+
 int rtm_autotune(rtm_ctx* c, int nsteps, int* zseg) {
   if (!c || nsteps < 1) return RTM_E_ARG;
   if (!c->model_ok || !c->wav_ok || !c->shots_ok)
     return fail(c, RTM_E_STATE, "model, wavelet and shots must be set");
   static const int kCand[] = {8, 16, 24, 32, 48, 64, 96};
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-    if (e0) (void)hipEventDestroy(e0);
-    return fail(c, RTM_E_HIP, "hipEventCreate failed");
-  }
-  const int saved_timing = c->timing;
-  c->timing = 0;
-  int rc = rtm_forward_begin(c, 0);  // defined fields to run the kernels on
-  // Each interior kernel kind of the migration is timed alone over nsteps
-  // launches per candidate on the batch's own buffers; the fields it writes
-  // are scratch here (the passes are begun anew afterwards).  Born and the
-  // adjoint share the receiver step's register-queue shape and take its
-  // choice.
+  // Each interior kernel kind of the migration is timed where it runs: inside
+  // forward and backward step pairs on the batch's own buffers (the frame,
+  // strip and imaging kernels interleaved, the buffers rotating), one kind at a
+  // time, the others held at their current choice.  Timed alone, back to back
+  // on the same buffers, the receiver step preferred 8-row segments; inside
+  // the step sequence 32 rows ran 13% faster.  The fields written here are
+  // scratch (the passes are begun anew afterwards); the timing counters of
+  // rtm_kernel_time are reset.
+  const int saved_timing = c->timing, saved_check = c->check_every;
+  c->check_every = -1;
+  int rc = rtm_forward_begin(c, 1);
+  if (rc == RTM_OK && !c->data_ok)
+    rc = launch_fill(c->d_data, (long long)c->nshot * c->p.nt * c->p.n, 0.f, c->stream) == hipSuccess
+             ? RTM_OK : fail(c, RTM_E_HIP, "autotune: fill");
   const int fwd = c->p.ilste ? EPI_FWD64 : EPI_FWD;
-  const int kinds[5] = {fwd, EPI_SRCI, EPI_RCVI, EPI_SRC, EPI_RCV};
-  for (int ki = 0; ki < 5 && rc == RTM_OK; ++ki) {
-    const int epi = kinds[ki];
-    int best = c->zs;
-    float best_ms = 3.0e38f;
+  const bool split = c->p.icompen == 1;
+  const int kinds[3] = {fwd, split ? EPI_SRCI : EPI_SRC, split ? EPI_RCVI : EPI_RCV};
+  const int timers[3] = {RTM_K_FWD_INTERIOR, split ? RTM_K_SRCI_INTERIOR : RTM_K_SRC_INTERIOR,
+                         split ? RTM_K_RCVI_INTERIOR : RTM_K_RCV_INTERIOR};
+  for (int ki = 0; ki < 3; ++ki)
+    if (c->zs_epi[kinds[ki]] <= 0) c->zs_epi[kinds[ki]] = 32;
+  const int nt = c->p.nt;
+  int kf = 2, kb = nt - 3;
+  auto pairs = [&](int n) -> int {
+    for (int i = 0; i < n; ++i) {
+      int r = forward_step(c, kf);
+      if (r == RTM_OK) r = backward_step(c, kb);
+      if (r != RTM_OK) return r;
+      kf = kf + 1 < nt ? kf + 1 : 2;
+      kb = kb > 0 ? kb - 1 : nt - 3;
+    }
+    return RTM_OK;
+  };
+  for (int ki = 0; ki < 3 && rc == RTM_OK; ++ki) {
+    const int epi = kinds[ki], kind = timers[ki];
+    int best = c->zs_epi[epi];
+    double best_ms = 1e300;
     for (int z : kCand) {
-      IntArgs a;
-      fill_int_args(c, a);
-      const bool rcv = epi == EPI_RCV || epi == EPI_RCVI;
-      a.u1 = rcv ? c->Q[1] : c->F[1]; a.u0 = rcv ? c->Q[0] : c->F[0];
-      a.u2 = rcv ? c->Q[2] : c->F[2];
-      a.uk = c->F[1];
-      a.wav = 0.f; a.k = 2; a.zs = z;
-      hipError_t e = launch_interior(epi, c->nfd, a, c->nshot, c->stream);  // untimed first
-      if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
-      for (int i = 0; i < nsteps && e == hipSuccess; ++i)
-        e = launch_interior(epi, c->nfd, a, c->nshot, c->stream);
-      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      if (e != hipSuccess) { rc = fail(c, RTM_E_HIP, "autotune timing: %s", hipGetErrorString(e)); break; }
+      c->zs_epi[epi] = z;
+      c->timing = 0;
+      rc = pairs(1);  // untimed
+      if (rc != RTM_OK) break;
+      c->timing = 1 << kind;
+      c->t_ms[kind] = 0.0;
+      c->t_n[kind] = 0;
+      rc = pairs(nsteps);
+      if (rc == RTM_OK) rc = drain_timing(c);
+      if (rc != RTM_OK) break;
+      const double ms = c->t_n[kind] ? c->t_ms[kind] / c->t_n[kind] : 1e300;
       if (ms < best_ms) { best_ms = ms; best = z; }
     }
     c->zs_epi[epi] = best;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   c->timing = saved_timing;
-  c->zs_epi[EPI_BORN] = c->zs_epi[EPI_BORN64] = c->zs_epi[EPI_ADJ] = c->zs_epi[EPI_RCV];
+  c->check_every = saved_check;
+  for (int i = 0; i < RTM_K_NKINDS; ++i) { c->t_ms[i] = 0.0; c->t_n[i] = 0; }
+  if (split) {  // checkpoint steps run the unsplit kernels of the same shape
+    c->zs_epi[EPI_SRC] = c->zs_epi[EPI_SRCI];
+    c->zs_epi[EPI_RCV] = c->zs_epi[EPI_RCVI];
+  }
+  c->zs_epi[EPI_BORN] = c->zs_epi[EPI_BORN64] = c->zs_epi[EPI_ADJ] = c->zs_epi[kinds[2]];
   c->zs = c->zs_epi[fwd];
   c->kf = -1;
   c->kb = -2;
   drop_graph(c);
   if (zseg) {  // the kernels of an ordinary migration step
     zseg[0] = c->zs_epi[fwd];
-    zseg[1] = c->zs_epi[EPI_SRCI];
-    zseg[2] = c->zs_epi[EPI_RCVI];
+    zseg[1] = c->zs_epi[kinds[1]];
+    zseg[2] = c->zs_epi[kinds[2]];
   }
   return rc;
 }

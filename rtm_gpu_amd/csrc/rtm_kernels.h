@@ -29,6 +29,12 @@ struct Model {
   const float* w;
   const float* refl;   // reflectivity (Born modelling), padded layout, per shot at rstride
   long long rstride;   // 0: one reflectivity field for all shots
+  // Neighbour-tap planes of the transposed stencil (per-cell rows): plane
+  // 4*(l-1)+d holds, per cell, the tap-l coefficient of the neighbour's own row,
+  // d = 0 above (z-l), 1 below (z+l), 2 left (x-l), 3 right (x+l); one set of
+  // 4*NFD planes (fstride each) per model window at tpstride (0: shared).
+  const float* tp;
+  long long tpstride;
   int cs, nrow;
   float hzx2_1, tao2, h2, taoh, taoh2;
 };
@@ -72,7 +78,7 @@ struct Snaps {
 // reconstruction with the source-side image terms (sum u, I2 += u*u) and the
 // receiver step with the rest (sum q, I1; compensated imaging only).
 enum { EPI_FWD = 0, EPI_SRC = 1, EPI_RCV = 2, EPI_FWD64 = 3, EPI_BORN = 4, EPI_BORN64 = 5,
-       EPI_ADJ = 6, EPI_SRCI = 7, EPI_RCVI = 8 };
+       EPI_ADJ = 6, EPI_SRCI = 7, EPI_RCVI = 8, EPI_ADJT = 9 };
 enum { FR_FWD = 0, FR_RCV = 1, FR_BORN = 2, FR_ADJ = 3 };
 
 struct IntArgs {
@@ -118,6 +124,7 @@ struct FrameArgs {
   Strips st;
   int save_strips;
   const float *ua, *ub, *uc;  // Born background fields (as in IntArgs)
+  int adjt;            // adjoint with per-cell rows: interior cells take the transposed stencil
 };
 
 // Launchers (rtm_kernels.hip).  nfd_inst is the compile-time stencil
@@ -134,6 +141,10 @@ hipError_t launch_frame(int mode, int nfd_inst, FrameArgs a, int nshot, int nrec
 void frame_records(const Grid& g, int nx, int shot, int seg, int4* out, int* count);
 int frame_segment_rows(int nfd);
 hipError_t launch_fill(float* p, long long n, float val, hipStream_t s);
+// Neighbour-tap planes (Model::tp) of one model window from its coefficient
+// offsets ko (padded layout) and the coefficient table: 4*nfd planes at tp.
+hipError_t launch_tap_planes(const Grid& g, const unsigned* ko, const float* ctab, int nfd,
+                             float* tp, hipStream_t s);
 // Finite-value watch: reset *flag to 0xffffffff; store k in *flag (if still
 // 0xffffffff) when any of the n floats at f (n a multiple of 4, 16-byte
 // aligned) is a NaN or an infinity.

@@ -246,7 +246,8 @@ struct EpiTraits {
   static constexpr bool rcvi = EPI == EPI_RCVI;  // ... without the source-side terms
   static constexpr bool srci = EPI == EPI_SRCI;  // reconstruction + source-side terms
   static constexpr bool born = EPI == EPI_BORN || EPI == EPI_BORN64;
-  static constexpr bool adj = EPI == EPI_ADJ;
+  static constexpr bool adj = EPI == EPI_ADJ || EPI == EPI_ADJT;
+  static constexpr bool adjt = EPI == EPI_ADJT;  // transposed stencil (per-cell rows)
   static constexpr bool src = !rcv && !born && !adj;  // injects the source wavelet
   static constexpr int PD = (rcv || born || adj) ? 1 : 2;  // prefetch distance of the stencil operands
 };
@@ -321,6 +322,8 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   const Rsrc RC = rsrc(m.ctab, (unsigned)(m.nrow * m.cs) * 4u);
   const unsigned P4 = (unsigned)g.pitch * 4u;
   auto rowb = [&](int z) -> unsigned { return (unsigned)min(z, g.nz - 1) * P4; };
+  Rsrc RTP;
+  if (T::adjt) RTP = rsrc(m.tp + (long long)shot * m.tpstride, (unsigned)(4 * NFD) * fb);
 
   const int2 sp = make_int2(sg.r_u, sg.r_x);
   Rcv rr = a.r;  // this shot's receivers; a.r.n stays the stride of the data layout
@@ -388,12 +391,23 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
 #pragma unroll
     for (int j = 0; j < QL; ++j) {
       const int zz = z + j;
-      // (a) coefficient rows of this row
-      CoefRow<NFD> cr[4];
-      cr[0].load(RC, kb[0].x);
-      cr[1].load(RC, kb[0].y);
-      cr[2].load(RC, kb[0].z);
-      cr[3].load(RC, kb[0].w);
+      // (a) coefficient rows of this row (transposed stencil: the cells' own
+      // c0' and the neighbour-tap planes of the row)
+      CoefRow<T::adjt ? 1 : NFD> cr[4];
+      float4 tpv[T::adjt ? 4 * NFD : 1];
+      if (T::adjt) {
+        cr[0].r[0] = bld4(RC, kb[0].x, 0);
+        cr[1].r[0] = bld4(RC, kb[0].y, 0);
+        cr[2].r[0] = bld4(RC, kb[0].z, 0);
+        cr[3].r[0] = bld4(RC, kb[0].w, 0);
+#pragma unroll
+        for (int t = 0; t < 4 * NFD; ++t) tpv[t] = bld4(RTP, vcol, rowb(zz) + (unsigned)t * fb);
+      } else {
+        cr[0].load(RC, kb[0].x);
+        cr[1].load(RC, kb[0].y);
+        cr[2].load(RC, kb[0].z);
+        cr[3].load(RC, kb[0].w);
+      }
       __builtin_amdgcn_sched_barrier(0);
       // Receiver step: this row's imaging operands, issued ahead of the
       // prefetches so that their wait leaves the prefetches in flight (a
@@ -440,12 +454,26 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
       for (int jj = 0; jj < 4; ++jj) {
         const float u = compr(qc, jj);
         float w1 = (float)(cr[jj].c0d() * (double)u);
+        if (T::adjt) {  // each tap with the neighbour's coefficient: (L^T mu)
 #pragma unroll
-        for (int l = 1; l <= NFD; ++l) {
-          float s = compr(q[(j + NFD - l) % QL], jj) + compr(q[(j + NFD + l) % QL], jj);
-          s = __builtin_fmaf(s, m.hzx2_1, compr(win[(H + jj - l) >> 2], (H + jj - l) & 3));
-          s = s + compr(win[(H + jj + l) >> 2], (H + jj + l) & 3);
-          w1 = __builtin_fmaf(cr[jj].c(l), s, w1);
+          for (int l = 1; l <= NFD; ++l) {
+            const float cu = compr(tpv[4 * (l - 1)], jj), cd = compr(tpv[4 * (l - 1) + 1], jj);
+            const float cl = compr(tpv[4 * (l - 1) + 2], jj), cr2 = compr(tpv[4 * (l - 1) + 3], jj);
+            const float vs = __builtin_fmaf(cu, compr(q[(j + NFD - l) % QL], jj),
+                                            cd * compr(q[(j + NFD + l) % QL], jj));
+            float s = __builtin_fmaf(vs, m.hzx2_1,
+                                     cl * compr(win[(H + jj - l) >> 2], (H + jj - l) & 3));
+            s = __builtin_fmaf(cr2, compr(win[(H + jj + l) >> 2], (H + jj + l) & 3), s);
+            w1 = w1 + s;
+          }
+        } else {
+#pragma unroll
+          for (int l = 1; l <= NFD; ++l) {
+            float s = compr(q[(j + NFD - l) % QL], jj) + compr(q[(j + NFD + l) % QL], jj);
+            s = __builtin_fmaf(s, m.hzx2_1, compr(win[(H + jj - l) >> 2], (H + jj - l) & 3));
+            s = s + compr(win[(H + jj + l) >> 2], (H + jj + l) & 3);
+            w1 = __builtin_fmaf(cr[jj].c(l), s, w1);
+          }
         }
         float o = combine_p(u, compr(u0v, jj), compr(pv, jj), w1, T::f64);
         if (T::src)       // source (kernel.cu:74-77, 277-280)
@@ -637,6 +665,12 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
         : a.st.stage + ((long long)shot * 2 + (a.k & 1)) * ss;
     RS = rsrc(sb, a.save_strips ? (unsigned)ss * 4u : 0u);
   }
+  // adjoint with per-cell rows: the interior cells of the march (the rows and
+  // columns next to the frame, whose pre-blend values the one-way update
+  // reads) take the transposed stencil of the interior kernel
+  const bool adjt = MODE == FR_ADJ && a.adjt != 0;
+  Rsrc RTP;
+  if (MODE == FR_ADJ) RTP = rsrc(m.tp + (long long)shot * m.tpstride, adjt ? (unsigned)(4 * NFD) * fb : 0u);
   Rsrc RBX, RBA, RBB, RBC;
   if (MODE == FR_BORN) {
     RBX = rsrc(m.refl + (long long)shot * m.rstride, fb);
@@ -745,6 +779,23 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
         s = __builtin_fmaf(s, m.hzx2_1, hl[l - 1]);
         s = s + hr[l - 1];
         w1 = __builtin_fmaf(crc.c(l), s, w1);
+      }
+      if (MODE == FR_ADJ) {  // (L^T mu) on interior cells, each tap with the neighbour's row
+        const bool inner = tt >= N2 && tt < NZ - N2 && x >= N2 && x < NX - N2;
+        float w1t = (float)(crc.c0d() * (double)u);
+#pragma unroll
+        for (int l = 1; l <= NFD; ++l) {
+          const unsigned rz = rowm(tt);
+          const float cu = bld1(RTP, vcol, rz + (unsigned)(4 * (l - 1)) * fb);
+          const float cd = bld1(RTP, vcol, rz + (unsigned)(4 * (l - 1) + 1) * fb);
+          const float cl = bld1(RTP, vcol, rz + (unsigned)(4 * (l - 1) + 2) * fb);
+          const float cr2 = bld1(RTP, vcol, rz + (unsigned)(4 * (l - 1) + 3) * fb);
+          const float vs = __builtin_fmaf(cu, q[(j + NB - l + U) % U], cd * q[(j + NB + l) % U]);
+          float sv = __builtin_fmaf(vs, m.hzx2_1, cl * hl[l - 1]);
+          sv = __builtin_fmaf(cr2, hr[l - 1], sv);
+          w1t = w1t + sv;
+        }
+        w1 = (adjt && inner) ? w1t : w1;
       }
       const float u0 = u0b[(j + 2) % 6], pmv = pmb[j % 3];
       const float* ex = ext[j % 2];
@@ -1244,7 +1295,7 @@ __global__ void k_stack_finish(int nI, const float* s1, const float* s2, int nre
 
 int interior_queue_rows(int epi, int nfd) {
   const bool pd1 = epi == EPI_RCV || epi == EPI_RCVI || epi == EPI_BORN || epi == EPI_BORN64 ||
-                   epi == EPI_ADJ;
+                   epi == EPI_ADJ || epi == EPI_ADJT;
   return 2 * nfd + 1 + (pd1 ? 1 : 2);  // EpiTraits<EPI>::PD
 }
 
@@ -1267,6 +1318,7 @@ hipError_t launch_interior(int epi, int nfd, IntArgs a, int nshot, hipStream_t s
     else if (epi == EPI_BORN) hipLaunchKernelGGL((k_interior<N, EPI_BORN>), grid, block, 0, s, a); \
     else if (epi == EPI_BORN64) hipLaunchKernelGGL((k_interior<N, EPI_BORN64>), grid, block, 0, s, a); \
     else if (epi == EPI_ADJ) hipLaunchKernelGGL((k_interior<N, EPI_ADJ>), grid, block, 0, s, a); \
+    else if (epi == EPI_ADJT) hipLaunchKernelGGL((k_interior<N, EPI_ADJT>), grid, block, 0, s, a); \
     else if (epi == EPI_SRCI) hipLaunchKernelGGL((k_interior<N, EPI_SRCI>), grid, block, 0, s, a); \
     else if (epi == EPI_RCVI) hipLaunchKernelGGL((k_interior<N, EPI_RCVI>), grid, block, 0, s, a); \
     else hipLaunchKernelGGL((k_interior<N, EPI_RCV>), grid, block, 0, s, a);               \
@@ -1324,6 +1376,33 @@ static inline unsigned blocks_for(long long n, int per = 256, unsigned cap = 409
   return (unsigned)(b > cap ? cap : b);
 }
 
+// Neighbour-tap planes: for cell (z, c) of the padded layout (c the storage
+// column) and l = 1..nfd, the tap-l coefficient of the rows of (z-l, c),
+// (z+l, c), (z, c-l), (z, c+l) -- 0 off the grid.  A row holds [c0' (double) |
+// c_1 .. c_NFD | zeros], so tap l sits at float 1 + l of the row.
+__global__ void k_tap_planes(Grid g, const unsigned* ko, const float* ctab, int nfd, float* tp) {
+  const long long n = (long long)g.nz * g.pitch;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int z = (int)(i / g.pitch), c = (int)(i - (long long)z * g.pitch);
+    for (int l = 1; l <= nfd; ++l) {
+      const int zz[4] = {z - l, z + l, z, z};
+      const int cc[4] = {c, c, c - l, c + l};
+      for (int d = 0; d < 4; ++d) {
+        float val = 0.0f;
+        if (zz[d] >= 0 && zz[d] < g.nz && cc[d] >= 0 && cc[d] < g.pitch)
+          val = ctab[ko[(long long)zz[d] * g.pitch + cc[d]] / 4u + 1 + l];
+        tp[(long long)(4 * (l - 1) + d) * g.fstride + i] = val;
+      }
+    }
+  }
+}
+hipError_t launch_tap_planes(const Grid& g, const unsigned* ko, const float* ctab, int nfd,
+                             float* tp, hipStream_t s) {
+  hipLaunchKernelGGL(k_tap_planes, dim3(blocks_for((long long)g.nz * g.pitch, 256, 2048)),
+                     dim3(256), 0, s, g, ko, ctab, nfd, tp);
+  return hipGetLastError();
+}
 hipError_t launch_flag_reset(unsigned* flag, hipStream_t s) {
   hipLaunchKernelGGL(k_flag_reset, dim3(1), dim3(64), 0, s, flag);
   return hipGetLastError();

@@ -96,6 +96,8 @@ def lib():
         L.rtm_stream_copy.argtypes = [vp, vp, C.c_longlong, vp]
         L.rtm_stream_copy.restype = C.c_int
         L.rtm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+        L.rtm_set_zseg.argtypes = [vp, C.POINTER(C.c_int)]
+        L.rtm_set_zseg.restype = C.c_int
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
                   "rtm_set_shot_geometry", "rtm_set_shot_model",
@@ -306,6 +308,11 @@ class Context:
         z = (C.c_int * 3)()
         self._chk(lib().rtm_autotune(self.h, int(nsteps), z), "rtm_autotune")
         return list(z)
+
+    def set_zseg(self, fwd, src, rcv):
+        """Rows per wave of the forward, reconstruction and receiver-step kernels."""
+        z = (C.c_int * 3)(int(fwd), int(src), int(rcv))
+        self._chk(lib().rtm_set_zseg(self.h, z), "rtm_set_zseg")
 
     def set_timing(self, mask):
         """mask: bit set of kernel kinds (1 << K_*), 0 off, -1 all."""

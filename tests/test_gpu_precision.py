@@ -10,10 +10,10 @@ same checkpoints (kept exactly):
   k = 150 (50 steps below the first checkpoint pair, not itself restored);
   and the final image accumulator I1.
 
-Default size: the faulted reverse-VSP model at 120 x 400, 1000 steps.  With
-RTM_PRECISION_FULL=1 the 351 x 1701, 4000-step geometry runs (about a minute
-of FP64 on 16 host threads); RTM_PRECISION_OUT=<path> writes the table as
-markdown."""
+Default size: the reverse-VSP geometry, 351 x 1701 interior, 4000 steps,
+nfdmax 4 (about a minute of FP64 on 16 host threads).  RTM_PRECISION_SMALL=1
+runs the faulted model at 120 x 400, 1000 steps instead;
+RTM_PRECISION_OUT=<path> writes the table as markdown."""
 import os
 
 import numpy as np
@@ -23,7 +23,7 @@ import cases
 
 pytestmark = pytest.mark.gpu
 
-FULL = os.environ.get("RTM_PRECISION_FULL") == "1"
+FULL = os.environ.get("RTM_PRECISION_SMALL") != "1"
 SNAP = 200
 FORMATS = {0: "FP32", 1: "FP16", 2: "BF16"}
 
