@@ -28,8 +28,8 @@ typedef struct rtm_cnn rtm_cnn;
 /* channels[0..nlayer]: input channels, hidden widths, 1 output channel; every
  * width 1..32, nlayer 1..16.  dilation[0..nlayer-1]: per-layer dilation 1..64
  * of the 3x3 taps (NULL: all 1; zero padding of the same width).  max_pixels
- * bounds n*h*w of a batch, max_rows bounds n*h.  Parameters start at zero
- * (the network then passes x[0]). */
+ * (at most 2^25) bounds n*h*w of a batch, max_rows (at most max_pixels)
+ * bounds n*h.  Parameters start at zero (the network then passes x[0]). */
 int rtm_cnn_create(int nlayer, const int* channels, const int* dilation, long long max_pixels,
                    long long max_rows, rtm_cnn** out);
 int rtm_cnn_destroy(rtm_cnn* net);

@@ -96,10 +96,10 @@ class Deblur:
         self._chk(_lib().rtm_cnn_get_params(self.h, p), "rtm_cnn_get_params")
         return p
 
-    def save(self, path):
-        """Parameters, widths and dilations to a .npz (plain arrays)."""
+    def save(self, path, **extra):
+        """Parameters, widths and dilations (and any extra plain arrays) to a .npz."""
         np.savez(path, params=self.params(), channels=np.asarray(self.channels, np.int32),
-                 dilation=np.asarray(self.dilation, np.int32))
+                 dilation=np.asarray(self.dilation, np.int32), **extra)
 
     @classmethod
     def load(cls, path, max_pixels, max_rows):

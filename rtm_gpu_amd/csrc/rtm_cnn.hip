@@ -3,34 +3,35 @@
 // connection from input channel 0, trained with Adam on mean-squared error.
 // Layers may be dilated (tap offsets times the layer's dilation), which
 // widens the receptive field at the same cost.
-// Convolutions run on the matrix cores: operands in BF16, accumulation in
-// FP32 (v_mfma_f32_16x16x16_bf16); master weights, Adam state, loss and
-// gradient reductions stay FP32 / FP64.
+// Convolutions run on gfx950's matrix cores: operands in BF16, accumulation
+// in FP32 (v_mfma_f32_16x16x32_bf16, K = 32 per instruction); master
+// weights, Adam state, loss and gradient reductions stay FP32 / FP64.
 //
 // Layouts.  Activations are NHWC in BF16 with the channel count padded to a
-// multiple of 4 (Cp), so the four consecutive K values a lane feeds the MFMA
-// are one 8-byte load.  Convolution as an implicit GEMM per wave: M = 16
+// multiple of 8 (Cp), so the eight consecutive K values a lane feeds the
+// MFMA are one 16-byte load.  Convolution as an implicit GEMM: M = 16
 // consecutive pixels of one row, N = 16 output channels, K = 9 taps x Cp
-// input channels (k = tap*Cp + ci, padded to a multiple of 16).  The MFMA
-// lane layout: A[m][k] at lane m + 16*(k/4), element k%4; B[k][n] at lane
-// n + 16*(k/4); D[m][n] at lane n + 16*(m/4), element m%4.
+// input channels (k = tap*Cp + ci, padded to a multiple of 32).  The MFMA
+// lane layout: A[m][k] at lane m + 16*(k/8), element k%8; B[k][n] at lane
+// n + 16*(k/8), element k%8; D[m][n] at lane n + 16*(m/4), element m%4.
 //
 //   forward      z = conv(a, W) + b; hidden layers store relu(z) in BF16,
 //                the last layer y = z + x0 in FP32
 //   input grad   dz_prev = conv(dz, W flipped and transposed) * (a > 0)
 //   weight grad  dW[tap][co][ci] = sum_pix dz[pix][co] * a[pix + tap][ci]:
-//                M = co, N = ci, K = pixels; one wave per image row and
-//                16 x 16 channel block writes its partial sums, a second
-//                kernel adds them in row order (FP64), so the result does not
-//                depend on scheduling.
-// Channel counts above 16 run in blocks of 16 output channels (one wave per
-// block) and, for the weight gradient, blocks of 16 input channels.
+//                M = co, N = ci, K = pixels, staged transposed through LDS;
+//                each workgroup accumulates a range of rows and writes one
+//                partial, two reduction kernels add the partials in a fixed
+//                order (FP64), so the result does not depend on scheduling.
+// Channel counts above 16 run in blocks of 16 (two blocks of output channels
+// per convolution wave; four 16 x 16 blocks per weight-gradient workgroup).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <string>
 #include <vector>
@@ -39,13 +40,19 @@
 
 namespace {
 
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
 constexpr int MAXC = 32;                    // channels per layer (two blocks of 16)
 constexpr int MAXL = 16;                    // layers
-constexpr int WG_STRIDE = 9 * MAXC * MAXC + MAXC;  // weight-grad partials per row
+constexpr int WG_STRIDE = 9 * MAXC * MAXC + MAXC;  // weight-grad partials per workgroup
+constexpr int WG_GROUPS = 1024;             // most workgroups of one weight-gradient launch
+constexpr int WG_LEVEL1 = 32;               // groups of the first reduction level
+// Byte offsets into an activation (MAX_PIXELS x 32 channels x 2 bytes) stay
+// below 2^31, the range of a buffer load's offset.
+constexpr long long MAX_PIXELS = 1LL << 25;
 
 __device__ __forceinline__ bf16 to_bf16(float f) {  // round to nearest even
   unsigned u = __float_as_uint(f);
@@ -54,20 +61,24 @@ __device__ __forceinline__ bf16 to_bf16(float f) {  // round to nearest even
 }
 __device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float(((unsigned)h) << 16); }
 
-__device__ __forceinline__ s4 load4(const bf16* p) {  // 8-byte aligned
-  const uint2 v = *reinterpret_cast<const uint2*>(p);
-  s4 r;
-  r[0] = (short)(v.x & 0xffffu); r[1] = (short)(v.x >> 16);
-  r[2] = (short)(v.y & 0xffffu); r[3] = (short)(v.y >> 16);
-  return r;
+
+// Buffer resources: a load whose offset is out of range (kDrop) returns zero,
+// which is the convolution's zero padding without a branch around the load.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr unsigned kDrop = 0x80000000u;
+__device__ __forceinline__ Rsrc rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ u32x4 bld16(Rsrc r, unsigned vo) {
+  return __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, 0, 0);
 }
 
 // mode 0: hidden layer (bias, ReLU, BF16 NHWC out); mode 1: last layer (bias
 // + residual, FP32 out channel 0); mode 2: input gradient (mask by act > 0).
 struct ConvArgs {
   const bf16* in;     // NHWC, cin_p channels
-  const bf16* wt;     // [16][kpad]
-  const float* bias;  // [16] (modes 0, 1)
+  const bf16* wt;     // [MAXC][kpad]
+  const float* bias;  // [MAXC] (modes 0, 1)
   bf16* out;          // NHWC, cout_p channels (modes 0, 2)
   float* out32;       // [n][h][w] (mode 1)
   const float* resid; // [n][h][w] (mode 1)
@@ -77,132 +88,202 @@ struct ConvArgs {
   int dil;            // dilation: tap (ky, kx) reads (y + (ky-1)*dil, x + (kx-1)*dil)
 };
 
+// Weight-stationary convolution: a wave holds its B operands (all NK K-steps
+// of NCOB blocks of 16 output channels) in registers and walks 16-pixel row
+// tiles, so a tile costs NK 16-byte input loads per lane and NK * NCOB MFMAs.
+// K-step ks covers k = 32 ks .. 32 ks + 31; lane group kg feeds k = 32 ks +
+// 8 kg .. +7, eight consecutive channels of one tap (cin_p is a multiple of 8).
+template <int NK, int NCOB>
 __global__ __launch_bounds__(256) void k_conv(const ConvArgs a) {
   const int lane = threadIdx.x & 63;
-  const int segs = (a.w + 15) >> 4;
-  const long long wave_all = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const long long per = (long long)a.n * a.h * segs;
-  if (wave_all >= per * a.ncob) return;
-  const int cob = (int)(wave_all / per);  // block of 16 output channels
-  const long long wave = wave_all - cob * per;
-  const int seg = (int)(wave % segs);
-  const long long row = wave / segs;  // nn*h + y
-  const int y = (int)(row % a.h);
-  const long long nn = row / a.h;
   const int m = lane & 15, kg = lane >> 4;
-  const int x = seg * 16 + m;
-  const bf16* wrow = a.wt + (long long)(cob * 16 + m) * a.kpad;  // output channel of lane & 15
-  f4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < a.kpad; k0 += 16) {
-    const int k = k0 + 4 * kg;
-    const int tap = k / a.cin_p, ci = k - tap * a.cin_p;
-    s4 av = {0, 0, 0, 0};
-    if (tap < 9) {
-      const int yy = y + (tap / 3 - 1) * a.dil, xx = x + (tap % 3 - 1) * a.dil;
-      if (x < a.w && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
-        av = load4(a.in + ((nn * a.h + yy) * a.w + xx) * a.cin_p + ci);
-    }
-    const s4 bv = load4(wrow + k);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-  }
-  const int co = cob * 16 + (lane & 15);
+  b8 wv[NCOB][NK];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int px = seg * 16 + 4 * kg + j;
-    if (px >= a.w) continue;
-    const long long p = (nn * a.h + y) * a.w + px;
-    if (a.mode == 0) {
-      if (co < a.cout_p) a.out[p * a.cout_p + co] = to_bf16(fmaxf(acc[j] + a.bias[co], 0.0f));
-    } else if (a.mode == 1) {
-      if (co == 0) a.out32[p] = (acc[j] + a.bias[0]) + a.resid[p];
-    } else {
-      if (co < a.cout_p) {
-        const bool on = from_bf16(a.mask[p * a.cout_p + co]) > 0.0f;
-        a.out[p * a.cout_p + co] = to_bf16(on ? acc[j] : 0.0f);
+  for (int cb = 0; cb < NCOB; ++cb)
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks)
+      wv[cb][ks] = *reinterpret_cast<const b8*>(a.wt + (cb * 16 + m) * a.kpad + ks * 32 + 8 * kg);
+  int tdy[NK], tdx[NK], tci[NK];
+  bool tok[NK];
+#pragma unroll
+  for (int ks = 0; ks < NK; ++ks) {
+    const int k = ks * 32 + 8 * kg, tap = k / a.cin_p;
+    tok[ks] = tap < 9;
+    tdy[ks] = (tap / 3 - 1) * a.dil;
+    tdx[ks] = (tap % 3 - 1) * a.dil;
+    tci[ks] = k - tap * a.cin_p;
+  }
+  const unsigned hw = (unsigned)(a.h * a.w);
+  const Rsrc RI = rsrc(a.in, (unsigned)a.n * hw * (unsigned)a.cin_p * 2u);
+  const unsigned segs = (unsigned)(a.w + 15) >> 4;
+  const unsigned tiles = (unsigned)(a.n * a.h) * segs;
+  for (unsigned t = blockIdx.x * 4u + (threadIdx.x >> 6); t < tiles; t += gridDim.x * 4u) {
+    const unsigned row = t / segs, seg = t - row * segs;
+    const int y = (int)(row % (unsigned)a.h);
+    const unsigned nn = row / (unsigned)a.h;
+    const int x = (int)seg * 16 + m;
+    b8 av[NK];
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks) {
+      const int yy = y + tdy[ks], xx = x + tdx[ks];
+      const bool ok = tok[ks] && x < a.w && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+      const unsigned vo =
+          ok ? ((nn * hw + (unsigned)(yy * a.w + xx)) * (unsigned)a.cin_p + (unsigned)tci[ks]) * 2u
+             : kDrop;
+      av[ks] = __builtin_bit_cast(b8, bld16(RI, vo));
+    }
+    f4 acc[NCOB];
+#pragma unroll
+    for (int cb = 0; cb < NCOB; ++cb) acc[cb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks)
+#pragma unroll
+      for (int cb = 0; cb < NCOB; ++cb)
+        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[ks], wv[cb][ks], acc[cb], 0, 0, 0);
+    // D[pixel][channel]: lane & 15 = channel in the block, pixels 4 kg + j
+#pragma unroll
+    for (int cb = 0; cb < NCOB; ++cb) {
+      const int co = cb * 16 + m;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int px = (int)seg * 16 + 4 * kg + j;
+        if (px >= a.w) continue;
+        const long long p = (long long)row * a.w + px;
+        if (a.mode == 0) {
+          if (co < a.cout_p) a.out[p * a.cout_p + co] = to_bf16(fmaxf(acc[cb][j] + a.bias[co], 0.0f));
+        } else if (a.mode == 1) {
+          if (co == 0) a.out32[p] = (acc[cb][j] + a.bias[0]) + a.resid[p];
+        } else {
+          if (co < a.cout_p) {
+            const bool on = from_bf16(a.mask[p * a.cout_p + co]) > 0.0f;
+            a.out[p * a.cout_p + co] = to_bf16(on ? acc[cb][j] : 0.0f);
+          }
+        }
       }
     }
   }
 }
 
-// Weight / bias gradient partials of one image row per wave.
+// Weight / bias gradient partials of a range of image rows per workgroup.
 struct WgradArgs {
   const bf16* dz;   // NHWC, dz_p channels
   const bf16* act;  // NHWC, act_p channels (the layer input)
-  float* part;      // [rows][WG_STRIDE]
+  float* part;      // [groups][WG_STRIDE]
   int n, h, w, dz_p, act_p, dil;
+  int rows_per;     // image rows (over n*h) of one workgroup
 };
 
+constexpr int SROW = 40;  // LDS row stride (bf16) of a staged 32-pixel chunk
+
+// dW[tap][co][ci] = sum_pix dz[pix][co] act[pix + tap][ci] as MFMAs with
+// M = co, N = ci, K = 32 pixels of one row.  Per chunk the workgroup stages
+// dz (32 px x dz_p) and the nine tap-shifted act chunks (32 px x act_p)
+// transposed into LDS, so each lane's eight K values are one 16-byte LDS
+// read; wave w takes the 16 x 16 block (co block w & 1, ci block w >> 1) and
+// accumulates over all chunks of its rows in registers.  One partial per
+// workgroup; the reduction adds them in a fixed order.
 __global__ __launch_bounds__(256) void k_wgrad(const WgradArgs a) {
-  const int lane = threadIdx.x & 63;
-  const long long rows = (long long)a.n * a.h;
-  const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int ncob = (a.dz_p + 15) >> 4, ncib = (a.act_p + 15) >> 4;
-  if (wave >= rows * ncob * ncib) return;
-  const long long row = wave % rows;
-  const int blk = (int)(wave / rows);
-  const int cob = blk % ncob, cib = blk / ncob;  // 16 x 16 block of (co, ci)
-  const int y = (int)(row % a.h);
-  const long long nn = row / a.h;
-  const int c16 = lane & 15, kg = lane >> 4;
-  const int coA = cob * 16 + c16, ciB = cib * 16 + c16;
+  __shared__ __attribute__((aligned(16))) bf16 sdz[MAXC * SROW];
+  __shared__ __attribute__((aligned(16))) bf16 sact[9 * MAXC * SROW];
+  const int tid = threadIdx.x, lane = tid & 63, wvi = tid >> 6;
+  const int cob = wvi & 1, cib = wvi >> 1;
+  const int m = lane & 15, kg = lane >> 4;
+  const int R = a.n * a.h;
+  const int r0 = blockIdx.x * a.rows_per, r1 = min(R, r0 + a.rows_per);
+  for (int i = tid; i < MAXC * SROW; i += 256) sdz[i] = 0;  // padded channels stay zero
+  for (int i = tid; i < 9 * MAXC * SROW; i += 256) sact[i] = 0;
+  __syncthreads();
+  const bool active = cob * 16 < a.dz_p && cib * 16 < a.act_p;
   f4 acc[9];
 #pragma unroll
   for (int t = 0; t < 9; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
   float bsum = 0.0f;
-  for (int x0 = 0; x0 < a.w; x0 += 16) {
-    // A[m = co][k = pixel]: lane co = c16 holds pixels 4*kg + j
-    s4 av;
+  const int cvz = a.dz_p >> 3, cva = a.act_p >> 3;
+  const Rsrc RZ = rsrc(a.dz, (unsigned)R * (unsigned)a.w * (unsigned)a.dz_p * 2u);
+  const Rsrc RA = rsrc(a.act, (unsigned)R * (unsigned)a.w * (unsigned)a.act_p * 2u);
+  for (int r = r0; r < r1; ++r) {
+    const int y = r % a.h, nn = r / a.h;
+    for (int x0 = 0; x0 < a.w; x0 += 32) {
+      for (int i = tid; i < 32 * cvz; i += 256) {
+        const int px = i / cvz, cg = i - px * cvz;
+        const int xx = x0 + px;
+        const unsigned vo = xx < a.w ? ((unsigned)(r * a.w + xx) * (unsigned)a.dz_p + 8u * cg) * 2u
+                                     : kDrop;
+        const u32x4 v = bld16(RZ, vo);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int px = x0 + 4 * kg + j;
-      bf16 v = 0;
-      if (px < a.w && coA < a.dz_p) v = a.dz[((nn * a.h + y) * a.w + px) * a.dz_p + coA];
-      av[j] = (short)v;
-      bsum += from_bf16(v);
-    }
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-      const int yy = y + (t / 3 - 1) * a.dil;
-      // B[k = pixel][n = ci]: lane ci = c16 holds pixels 4*kg + j, shifted by the tap
-      s4 bv;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int px = x0 + 4 * kg + j;
-        const int xx = px + (t % 3 - 1) * a.dil;
-        bf16 v = 0;
-        if (px < a.w && ciB < a.act_p && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
-          v = a.act[((nn * a.h + yy) * a.w + xx) * a.act_p + ciB];
-        bv[j] = (short)v;
+        for (int j = 0; j < 8; ++j)
+          sdz[(8 * cg + j) * SROW + px] = (bf16)(v[j >> 1] >> (16 * (j & 1)));
       }
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+      for (int i = tid; i < 9 * 32 * cva; i += 256) {
+        const int tap = i / (32 * cva), q = i - tap * 32 * cva, px = q / cva, cg = q - px * cva;
+        const int yy = y + (tap / 3 - 1) * a.dil, xx = x0 + px + (tap % 3 - 1) * a.dil;
+        const bool ok = yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+        const unsigned vo =
+            ok ? ((unsigned)((nn * a.h + yy) * a.w + xx) * (unsigned)a.act_p + 8u * cg) * 2u : kDrop;
+        const u32x4 v = bld16(RA, vo);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          sact[(tap * MAXC + 8 * cg + j) * SROW + px] = (bf16)(v[j >> 1] >> (16 * (j & 1)));
+      }
+      __syncthreads();
+      if (active) {
+        const u32x4 araw = *reinterpret_cast<const u32x4*>(&sdz[(cob * 16 + m) * SROW + 8 * kg]);
+        const b8 av = __builtin_bit_cast(b8, araw);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+          const b8 bv = *reinterpret_cast<const b8*>(&sact[(t * MAXC + cib * 16 + m) * SROW + 8 * kg]);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+        }
+        if (cib == 0)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            bsum += __uint_as_float(araw[j] << 16) + __uint_as_float(araw[j] & 0xffff0000u);
+      }
+      __syncthreads();
     }
   }
-  float* P = a.part + row * WG_STRIDE;
+  if (!active) return;
+  float* P = a.part + (long long)blockIdx.x * WG_STRIDE;
+  const int ci = cib * 16 + m;
 #pragma unroll
   for (int t = 0; t < 9; ++t)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)  // D[m = co][n = ci]
-      P[(t * MAXC + cob * 16 + 4 * kg + j) * MAXC + ciB] = acc[t][j];
-  // bias: lanes c16, c16+16, c16+32, c16+48 hold pixels of the same channel
+    for (int j = 0; j < 4; ++j)  // D[m = co][n = ci]: co = cob*16 + 4 kg + j
+      P[(t * MAXC + cob * 16 + 4 * kg + j) * MAXC + ci] = acc[t][j];
+  // bias: lanes m, m+16, m+32, m+48 hold pixels of channel cob*16 + m
   bsum += __shfl_xor(bsum, 16);
   bsum += __shfl_xor(bsum, 32);
-  if (kg == 0 && cib == 0) P[9 * MAXC * MAXC + coA] = bsum;
+  if (kg == 0 && cib == 0) P[9 * MAXC * MAXC + cob * 16 + m] = bsum;
 }
 
-// grad[layer params] from the row partials: W[co][ci][ky][kx] then b[co].
-__global__ void k_wgrad_reduce(const float* part, long long rows, int cin, int cout, float* grad) {
+// Deterministic two-level sum of the workgroup partials into the layer's
+// gradient (W[co][ci][ky][kx] then b[co]): level 1 adds group g's share of
+// the partials in order (FP64), level 2 adds the groups in order.
+__device__ __forceinline__ int wgrad_src(int e, int cin, int cout) {
   const int nw = cout * cin * 9;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nw + cout) return;
-  int src;
   if (e < nw) {
     const int co = e / (cin * 9), r = e % (cin * 9), ci = r / 9, t = r % 9;
-    src = (t * MAXC + co) * MAXC + ci;
-  } else {
-    src = 9 * MAXC * MAXC + (e - nw);
+    return (t * MAXC + co) * MAXC + ci;
   }
+  return 9 * MAXC * MAXC + (e - nw);
+}
+
+__global__ void k_wgrad_reduce1(const float* part, int P, int G, int cin, int cout, double* part2) {
+  const int ne = cout * cin * 9 + cout;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  if (e >= ne) return;
+  const int src = wgrad_src(e, cin, cout);
+  const int p0 = (int)((long long)P * g / G), p1 = (int)((long long)P * (g + 1) / G);
   double s = 0.0;
-  for (long long r = 0; r < rows; ++r) s += (double)part[r * WG_STRIDE + src];
+  for (int p = p0; p < p1; ++p) s += (double)part[(long long)p * WG_STRIDE + src];
+  part2[(long long)g * ne + e] = s;
+}
+
+__global__ void k_wgrad_reduce2(const double* part2, int G, int ne, float* grad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double s = 0.0;
+  for (int g = 0; g < G; ++g) s += part2[(long long)g * ne + e];
   grad[e] = (float)s;
 }
 
@@ -227,6 +308,20 @@ __global__ void k_loss_grad(const float* y, const float* t, long long npix, int 
     const float d = y[p] - t[p];
     diff[p] = d;
     for (int c = 0; c < cp; ++c) dz[p * cp + c] = c == 0 ? to_bf16(scale * d) : (bf16)0;
+  }
+}
+
+// One training batch from the device-resident set: sample i of the batch is
+// set entry order[b0 + i] (inputs c0 channels, then the target).
+__global__ void k_gather_batch(const float* dsx, const float* dst, const int* order, int b0,
+                               int batch, long long img, int c0, float* x32, float* t32) {
+  const long long per = img * (c0 + 1);
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)batch * per;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long i = e / per, r = e - i * per;
+    const long long k = order[b0 + i];
+    if (r < img * c0) x32[i * img * c0 + r] = dsx[k * img * c0 + r];
+    else t32[i * img + (r - img * c0)] = dst[k * img + (r - img * c0)];
   }
 }
 
@@ -279,8 +374,8 @@ __global__ __launch_bounds__(256) void k_sumsq(const float* d, long long n, doub
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-inline int pad4(int c) { return (c + 3) & ~3; }
-inline int pad16(int k) { return (k + 15) & ~15; }
+inline int pad8(int c) { return (c + 7) & ~7; }
+inline int pad32(int k) { return (k + 31) & ~31; }
 inline unsigned nblk(long long n, int per = 256, long long cap = 8192) {
   long long b = (n + per - 1) / per;
   if (b < 1) b = 1;
@@ -308,13 +403,16 @@ struct rtm_cnn {
   bf16* act[MAXL] = {nullptr};  // act[l]: input of layer l, NHWC cp[l]
   bf16* dz[MAXL] = {nullptr};   // dz[l]: gradient at the output of layer l, NHWC cp[l+1]
   float *x32 = nullptr, *resid = nullptr, *y32 = nullptr, *t32 = nullptr, *diff = nullptr;
-  float* part = nullptr;
+  float* part = nullptr;    // weight-gradient partials [WG_GROUPS][WG_STRIDE]
+  double* part2 = nullptr;  // first reduction level [WG_LEVEL1][WG_STRIDE]
   double* sq = nullptr;
   int adam_t = 0;
   bool grad_ok = false;
   // device-resident training set
   float *ds_x = nullptr, *ds_t = nullptr;
+  int* ds_order = nullptr;
   long long ds_cap = 0;
+  int ds_n = 0;
 };
 
 namespace {
@@ -377,9 +475,26 @@ int pack_weights(rtm_cnn* c) {
   return RTM_CNN_OK;
 }
 
+template <int NK, int NCOB>
+void conv_go(const ConvArgs& a, unsigned blocks, hipStream_t s) {
+  hipLaunchKernelGGL((k_conv<NK, NCOB>), dim3(blocks), dim3(256), 0, s, a);
+}
+
 int launch_conv(rtm_cnn* c, const ConvArgs& a) {
-  const long long waves = (long long)a.n * a.h * ((a.w + 15) / 16) * a.ncob;
-  hipLaunchKernelGGL(k_conv, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, c->stream, a);
+  const long long tiles = (long long)a.n * a.h * ((a.w + 15) / 16);
+  const unsigned blocks = (unsigned)std::min<long long>((tiles + 3) / 4, 2048);
+  const int nk = a.kpad / 32;
+  switch (nk * 4 + a.ncob) {
+    case 3 * 4 + 1: conv_go<3, 1>(a, blocks, c->stream); break;
+    case 3 * 4 + 2: conv_go<3, 2>(a, blocks, c->stream); break;
+    case 5 * 4 + 1: conv_go<5, 1>(a, blocks, c->stream); break;
+    case 5 * 4 + 2: conv_go<5, 2>(a, blocks, c->stream); break;
+    case 7 * 4 + 1: conv_go<7, 1>(a, blocks, c->stream); break;
+    case 7 * 4 + 2: conv_go<7, 2>(a, blocks, c->stream); break;
+    case 9 * 4 + 1: conv_go<9, 1>(a, blocks, c->stream); break;
+    case 9 * 4 + 2: conv_go<9, 2>(a, blocks, c->stream); break;
+    default: return cfail(c, RTM_CNN_E_ARG, "no convolution for K %d, %d channel blocks", a.kpad, a.ncob);
+  }
   CNNCHK(c, hipGetLastError());
   return RTM_CNN_OK;
 }
@@ -429,14 +544,19 @@ int loss_grad_dev(rtm_cnn* c, const float* t_dev, int n, int h, int w, double* l
                      c->cp[L], (float)(2.0 / (double)npix), c->dz[L - 1], c->diff);
   CNNCHK(c, hipGetLastError());
   for (int l = L - 1; l >= 0; --l) {
-    WgradArgs g{c->dz[l], c->act[l], c->part, n, h, w, c->cp[l + 1], c->cp[l], c->dil[l]};
-    const long long rows = (long long)n * h;
-    const long long nwave = rows * ((c->cp[l + 1] + 15) / 16) * ((c->cp[l] + 15) / 16);
-    hipLaunchKernelGGL(k_wgrad, dim3((unsigned)((nwave + 3) / 4)), dim3(256), 0, c->stream, g);
+    const int rows = n * h;
+    const int rows_per = (rows + WG_GROUPS - 1) / WG_GROUPS;
+    const int P = (rows + rows_per - 1) / rows_per;
+    WgradArgs g{c->dz[l], c->act[l], c->part, n, h, w, c->cp[l + 1], c->cp[l], c->dil[l], rows_per};
+    hipLaunchKernelGGL(k_wgrad, dim3((unsigned)P), dim3(256), 0, c->stream, g);
     CNNCHK(c, hipGetLastError());
     const int ne = c->c[l + 1] * c->c[l] * 9 + c->c[l + 1];
-    hipLaunchKernelGGL(k_wgrad_reduce, dim3((ne + 63) / 64), dim3(64), 0, c->stream, c->part, rows,
-                       c->c[l], c->c[l + 1], c->grad + c->off[l]);
+    const int G = std::min(WG_LEVEL1, P);
+    hipLaunchKernelGGL(k_wgrad_reduce1, dim3((ne + 255) / 256, G), dim3(256), 0, c->stream, c->part,
+                       P, G, c->c[l], c->c[l + 1], c->part2);
+    CNNCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_wgrad_reduce2, dim3((ne + 255) / 256), dim3(256), 0, c->stream, c->part2,
+                       G, ne, c->grad + c->off[l]);
     CNNCHK(c, hipGetLastError());
     if (l > 0) {  // dz[l-1] = conv(dz[l], W^T flipped) * (act[l] > 0)
       ConvArgs a{};
@@ -494,17 +614,18 @@ int rtm_cnn_create(int nlayer, const int* channels, const int* dilation, long lo
     for (int l = 0; l < nlayer; ++l)
       if (dilation[l] < 1 || dilation[l] > 64) return RTM_CNN_E_ARG;
   if (channels[nlayer] != 1 || max_pixels < 1 || max_rows < 1) return RTM_CNN_E_ARG;
+  if (max_pixels > MAX_PIXELS || max_rows > max_pixels) return RTM_CNN_E_ARG;
   rtm_cnn* c = new (std::nothrow) rtm_cnn();
   if (!c) return RTM_CNN_E_NOMEM;
   c->L = nlayer;
   for (int l = 0; l <= nlayer; ++l) {
     c->c[l] = channels[l];
-    c->cp[l] = pad4(channels[l]);
+    c->cp[l] = pad8(channels[l]);
   }
   for (int l = 0; l < nlayer; ++l) {
     c->dil[l] = dilation ? dilation[l] : 1;
-    c->kf[l] = pad16(9 * c->cp[l]);
-    c->kb[l] = pad16(9 * c->cp[l + 1]);
+    c->kf[l] = pad32(9 * c->cp[l]);
+    c->kb[l] = pad32(9 * c->cp[l + 1]);
     c->off[l + 1] = c->off[l] + c->c[l + 1] * c->c[l] * 9 + c->c[l + 1];
   }
   c->np = c->off[nlayer];
@@ -528,7 +649,9 @@ int rtm_cnn_create(int nlayer, const int* channels, const int* dilation, long lo
   if (rc == RTM_CNN_OK) rc = calloc_dev(c, &c->y32, (size_t)max_pixels);
   if (rc == RTM_CNN_OK) rc = calloc_dev(c, &c->t32, (size_t)max_pixels);
   if (rc == RTM_CNN_OK) rc = calloc_dev(c, &c->diff, (size_t)max_pixels);
-  if (rc == RTM_CNN_OK) rc = calloc_dev(c, &c->part, (size_t)max_rows * WG_STRIDE);
+  if (rc == RTM_CNN_OK)
+    rc = calloc_dev(c, &c->part, (size_t)std::min<long long>(max_rows, WG_GROUPS) * WG_STRIDE);
+  if (rc == RTM_CNN_OK) rc = calloc_dev(c, &c->part2, (size_t)WG_LEVEL1 * WG_STRIDE);
   if (rc == RTM_CNN_OK) rc = calloc_dev(c, &c->sq, 256);
   if (rc == RTM_CNN_OK) {
     if (hipMemsetAsync(c->am, 0, c->np * sizeof(float), c->stream) != hipSuccess ||
@@ -625,15 +748,19 @@ int rtm_cnn_fit(rtm_cnn* c, const float* x, const float* t, int n, int h, int w,
   if (rc != RTM_CNN_OK) return rc;
   const size_t img = (size_t)h * w;
   const size_t need = (size_t)n * img;
-  if ((long long)need > c->ds_cap) {  // a larger training set replaces the old copy
+  if ((long long)need > c->ds_cap || n > c->ds_n) {  // a larger training set replaces the old copy
     CNNCHK(c, hipStreamSynchronize(c->stream));
     cfree_dev(c, &c->ds_x);
     cfree_dev(c, &c->ds_t);
+    cfree_dev(c, &c->ds_order);
     c->ds_cap = 0;
+    c->ds_n = 0;
     rc = calloc_dev(c, &c->ds_x, need * c->c[0]);
     if (rc == RTM_CNN_OK) rc = calloc_dev(c, &c->ds_t, need);
+    if (rc == RTM_CNN_OK) rc = calloc_dev(c, &c->ds_order, (size_t)n);
     if (rc != RTM_CNN_OK) return rc;
     c->ds_cap = (long long)need;
+    c->ds_n = n;
   }
   CNNCHK(c, hipMemcpyAsync(c->ds_x, x, need * c->c[0] * sizeof(float), hipMemcpyHostToDevice, c->stream));
   CNNCHK(c, hipMemcpyAsync(c->ds_t, t, need * sizeof(float), hipMemcpyHostToDevice, c->stream));
@@ -649,16 +776,15 @@ int rtm_cnn_fit(rtm_cnn* c, const float* x, const float* t, int n, int h, int w,
       const int j = (int)(rnd() % (unsigned)(i + 1));
       const int tmp = order[i]; order[i] = order[j]; order[j] = tmp;
     }
+    // the previous epoch ended with a stream synchronisation (the loss read-back)
+    CNNCHK(c, hipMemcpyAsync(c->ds_order, order.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
     double sum = 0.0;
     int nb = 0;
     for (int b0 = 0; b0 + batch <= n; b0 += batch) {
-      for (int i = 0; i < batch; ++i) {
-        const int k = order[b0 + i];
-        CNNCHK(c, hipMemcpyAsync(c->x32 + (size_t)i * img * c->c[0], c->ds_x + (size_t)k * img * c->c[0],
-                                 img * c->c[0] * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-        CNNCHK(c, hipMemcpyAsync(c->t32 + (size_t)i * img, c->ds_t + (size_t)k * img,
-                                 img * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-      }
+      hipLaunchKernelGGL(k_gather_batch, dim3(nblk((long long)batch * img * (c->c[0] + 1))), dim3(256), 0,
+                         c->stream, c->ds_x, c->ds_t, c->ds_order, b0, batch, (long long)img, c->c[0],
+                         c->x32, c->t32);
+      CNNCHK(c, hipGetLastError());
       rc = pack_input(c, c->x32, batch, h, w);
       double loss = 0.0;
       if (rc == RTM_CNN_OK) rc = loss_grad_dev(c, c->t32, batch, h, w, &loss);

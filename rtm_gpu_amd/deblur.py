@@ -6,26 +6,32 @@ The migrated image of Born data is H m = L^T L m: the reflectivity m blurred
 by the Hessian of Born modelling.  Least-squares migration (lsrtm.cgls)
 undoes the blur iteratively, at two wave-equation passes per iteration.  A
 small convolutional network (cnn.Deblur) learns an approximate inverse of H
-in the image domain instead:
+in the image domain instead, from pairs the engine makes itself:
+random reflectivity m_i -> Born data d_i = L m_i -> migrated image
+x_i = L^T d_i, many models per launch (each panel of a wide reflectivity is
+one group of shots imaging its own columns, with its own background window
+and its own CG step lengths).  Output m ~ c x + net(x) (c: the least-squares
+scalar between the training images and reflectivities; a zero network is the
+scaled migrated image).
 
-  training pairs   random reflectivities m_i (the high-pass velocity
-                   contrast (v^2 - v0^2)/v0^2 of faulted layered models drawn
-                   with other seeds than the evaluated one) -> Born data
-                   d_i = L m_i -> input x_i = L^T d_i = H m_i, target the
-                   image of cg_iters CGLS iterations on d_i (or m_i itself
-                   with target="true"), all by this engine, many pairs per
-                   launch (each panel of a wide reflectivity is one group of
-                   shots at its own image columns, with its own CG step
-                   lengths);
-  input channels   c * x (c: the least-squares scalar between images and
-                   reflectivities of the training set), the normalised log
-                   source illumination, and depth;
-  output           m ~ c x + net(x) (zero net: the scaled migrated image,
-                   the plain-migration fallback).
+Two settings:
 
-evaluate() compares the network's estimate with 10 CGLS iterations of the
-same engine on the evaluated model: relative L2 between the two images, the
-Born re-modelled data residual ||L m - d|| / ||d|| of each, and wall times."""
+  build_survey / run   one background (the smoothed faulted model) for every
+                       pair; inputs c x, log source illumination and depth;
+                       targets the CGLS images of the pairs or m itself.
+  cfg5_*               the random-reflectivity configuration: 1024 x 256 at
+                       10 m, every model with its own background (linear
+                       1500-4000 m/s gradient plus a Gaussian random field)
+                       and 10-40 dipping reflectors (synth.reflectors), 64
+                       surface shots, 25 Hz, 3000 steps; inputs c x, the
+                       normalised background velocity and depth; targets m;
+                       six 3x3 layers of 32 channels trained on 64 x 64
+                       patches, batch 256, Adam 1e-3; evaluated on held-out
+                       models against 10 CGLS iterations.
+
+guard() is the fallback check: the network's estimate is returned only when
+its Born re-modelled data fit the observed data better than both zero and the
+best multiple of the migrated image; otherwise that multiple is returned."""
 import time
 
 import numpy as np
@@ -200,100 +206,6 @@ def patches(XF, M, size, count, rng):
     return xs, ms
 
 
-def panel_residuals(ctx, panels, dm):
-    """||L dm_p|| per panel (dm [panels][mnz][mnx]) from one batched Born pass."""
-    op = lsrtm.BornOperator(ctx, groups=panels)
-    return np.sqrt(np.maximum(op.forward(dm), 0.0))
-
-
-def run_models(S, ntrain=96, neval=8, panels=8, epochs=100, patch=64, batch=256,
-               patches_per_epoch=2048, lr=1e-3, channels=(3,) + (32,) * 5 + (1,),
-               dilation=None, cg_iters=10, target="true", seed=0, save_net=""):
-    """Several random models: train on engine-made pairs (patches), evaluate on
-    held-out models against CGLS with per-model statistics."""
-    res = dict(grid=f"{S['mod_nz']}x{S['mod_nx']}", nt=int(S["d"].nt), shots=S["nshot"],
-               ntrain=ntrain, neval=neval, epochs=epochs, patch=patch, batch=batch,
-               channels=list(channels), dilation=list(dilation) if dilation else None,
-               target=target)
-    t = time.time()
-    X, M = make_pairs(S, ntrain, panels, seed0=1000, target=target, cg_iters=cg_iters)
-    res["pairs_s"] = time.time() - t
-    log(f"[deblur] {ntrain} training pairs in {res['pairs_s']:.1f}s")
-    c = float(np.sum(X.astype(np.float64) * M) / np.sum(X.astype(np.float64) ** 2))
-    res["scale"] = c
-    illum = illumination(S)
-    XF = features(S, X, c, illum)
-    del X
-    rng = np.random.default_rng(seed)
-    net = cnn.Deblur(list(channels), batch * patch * patch, batch * patch, dilation=dilation)
-    net.init_params(seed=seed)
-    t = time.time()
-    hist = []
-    for ep in range(epochs):  # fresh random crops each epoch
-        xp, mp = patches(XF, M, patch, patches_per_epoch, rng)
-        hist.extend(net.fit(xp, mp, epochs=1, batch=batch, lr=lr, seed=seed + ep))
-        if (ep + 1) % 10 == 0:
-            log(f"[deblur] epoch {ep + 1} of {epochs}: loss {hist[-1]:.3e}")
-    res["train_s"] = time.time() - t
-    res["loss_first"], res["loss_last"] = float(hist[0]), float(hist[-1])
-    if save_net:
-        net.save(save_net)
-    del XF, M
-    # held-out models: image, network, CGLS per panel
-    ctx = context(S, panels)
-    per = []
-    for b in range(0, neval, panels):
-        k = min(panels, neval - b)
-        mt = np.stack([random_reflectivity(S, 5000 + b + i) for i in range(panels)])
-        if k < panels:
-            mt[k:] = 0.0
-        normal_images(ctx, mt)  # Born data d of m into the gathers
-        t = time.time()  # the migrated image of d: a forward pass and the adjoint pass
-        ctx.forward_begin(True)
-        ctx.forward_advance(ctx.p.nt)
-        ctx.adjoint_begin()
-        ctx.adjoint_advance(ctx.p.nt)
-        g = ctx.adjoint_gradient()
-        img = np.ascontiguousarray(g.reshape(g.shape[0], panels, -1).transpose(1, 0, 2))
-        t_img = time.time() - t
-        xe = features(S, img, c, illum)
-        t = time.time()
-        m_net = np.stack([net.forward(xe[i:i + 1])[0] for i in range(panels)])
-        t_net = time.time() - t
-        t = time.time()
-        iters = _Iterates("[deblur] held-out CGLS iteration")
-        m_cg, _ = lsrtm.cgls(lsrtm.BornOperator(ctx, groups=panels), None, cg_iters,
-                             callback=iters)
-        t_cg = time.time() - t
-        reach = iters.first_within(m_cg, axis_models=True)
-        m_rtm = np.float32(c) * img
-        dn = panel_residuals(ctx, panels, mt)
-        rn = {name: panel_residuals(ctx, panels, est - mt) / np.where(dn > 0, dn, 1.0)
-              for name, est in (("rtm", m_rtm), ("net", m_net), ("cg", m_cg))}
-        for i in range(k):
-            per.append(dict(cg_iters_to_5pct=reach[i][0],
-                            rel_net_vs_cg=rel(m_net[i], m_cg[i]), rel_rtm_vs_cg=rel(m_rtm[i], m_cg[i]),
-                            rel_net_vs_true=rel(m_net[i], mt[i]), rel_cg_vs_true=rel(m_cg[i], mt[i]),
-                            resid_rtm=float(rn["rtm"][i]), resid_net=float(rn["net"][i]),
-                            resid_cg=float(rn["cg"][i])))
-        res.setdefault("t_image_s", t_img)
-        res.setdefault("t_net_s", t_net)
-        res.setdefault("t_cg_s", t_cg)
-        log(f"[deblur] held-out models {b}..{b + k - 1}: rel to CGLS net "
-            f"{np.mean([q['rel_net_vs_cg'] for q in per[-k:]]):.3f}")
-    for key in per[0]:
-        if key != "cg_iters_to_5pct":
-            res[key] = float(np.mean([q[key] for q in per]))
-    res["cg_iters_to_5pct"] = [q["cg_iters_to_5pct"] for q in per]
-    res["speedup_vs_cg"] = res["t_cg_s"] / (res["t_image_s"] + res["t_net_s"])
-    res["per_model"] = per
-    log(f"[deblur] {neval} held-out models: rel L2 to CGLS net {res['rel_net_vs_cg']:.3f}, "
-        f"scaled image {res['rel_rtm_vs_cg']:.3f}; data residual RTM {res['resid_rtm']:.3f}, "
-        f"net {res['resid_net']:.3f}, CGLS {res['resid_cg']:.3f}; time image+net "
-        f"{res['t_image_s'] + res['t_net_s']:.2f}s vs CGLS {res['t_cg_s']:.2f}s")
-    return res
-
-
 def born_residual(ctx, m, dobs):
     """Relative misfit ||L m - d|| / ||d|| of the Born re-modelled data of the
     reflectivity m (one Born pass on ctx; its records are overwritten)."""
@@ -302,20 +214,44 @@ def born_residual(ctx, m, dobs):
     return float(np.linalg.norm(r.astype(np.float64)) / np.linalg.norm(dobs.astype(np.float64)))
 
 
-def deblur(ctx, net, S, img, c, illum, dobs):
-    """The network's reflectivity estimate from the migrated image img, with
-    plain migration as the fallback: the network's output is returned only
-    when its Born re-modelled data fit the observed data dobs better than
-    those of the scaled image c * img, otherwise the scaled image is.  Costs
-    the network and two Born passes on ctx.  Returns (m, "net" or "rtm",
-    misfit of the network's estimate, misfit of the scaled image)."""
-    m_rtm = np.float32(c) * np.asarray(img, np.float32)
-    m_net = net.forward(features(S, img, c, illum))[0]
-    r_net = born_residual(ctx, m_net, dobs)
-    r_rtm = born_residual(ctx, m_rtm, dobs)
-    if np.isfinite(r_net) and r_net < r_rtm:
-        return m_net, "net", r_net, r_rtm
-    return m_rtm, "rtm", r_net, r_rtm
+def data_fit(op, m):
+    """One Born pass of m: (<L m, L m>, <L m, d>) per group, d the gathers."""
+    qq = np.atleast_1d(op.forward(m)).astype(np.float64)
+    qd = np.atleast_1d(op.dots(hip.DATA_RECORDS, hip.DATA_GATHERS)).astype(np.float64)
+    return qq, qd
+
+
+def guard(op, img, m_net):
+    """The fallback check, per group of op (lsrtm.BornOperator whose gathers
+    hold the observed data d): alpha = <L img, d> / ||L img||^2 scales the
+    migrated image img to its best data fit, misfit r_rtm =
+    ||alpha L img - d|| / ||d|| (at most 1, the misfit of zero); the network's
+    estimate m_net is kept only when its misfit r_net = ||L m_net - d|| / ||d||
+    is finite and below both r_rtm and 1.  Two Born passes (one if m_net is
+    not finite).  Returns (m, choice, r_net, r_rtm, alpha): arrays over the
+    groups when op has groups > 1, scalars otherwise."""
+    G = op.groups
+    img = np.asarray(img, np.float32)
+    m_net = np.asarray(m_net, np.float32)
+    dd = np.atleast_1d(op.dots(hip.DATA_GATHERS, hip.DATA_GATHERS)).astype(np.float64)
+    qq, qd = data_fit(op, img)
+    alpha = np.where(qq > 0, qd / np.where(qq > 0, qq, 1.0), 0.0)
+    safe = np.where(dd > 0, dd, 1.0)
+    r_rtm = np.sqrt(np.maximum(0.0, 1.0 - alpha * qd / safe))
+    fin = np.array([np.isfinite(g).all() for g in m_net.reshape(G, -1)])
+    r_net = np.full(G, np.inf)
+    if fin.any():
+        probe = np.where(fin.reshape((G,) + (1,) * (m_net.ndim - 1)) if G > 1 else fin[0],
+                         m_net, np.float32(0))
+        nq, nd = data_fit(op, probe)
+        r_net = np.where(fin, np.sqrt(np.maximum(0.0, nq - 2.0 * nd + dd) / safe), np.inf)
+    use = fin & (dd > 0) & (r_net < np.minimum(r_rtm, 1.0))
+    a32 = alpha.astype(np.float32)
+    if G == 1:
+        m = m_net if use[0] else a32[0] * img
+        return m, ("net" if use[0] else "rtm"), float(r_net[0]), float(r_rtm[0]), float(alpha[0])
+    m = np.where(use[:, None, None], m_net, a32[:, None, None] * img)
+    return m, np.where(use, "net", "rtm"), r_net, r_rtm, alpha
 
 
 def run(S=None, npairs=128, panels=32, epochs=600, batch=8, lr=1e-3,
@@ -368,15 +304,15 @@ def run(S=None, npairs=128, panels=32, epochs=600, batch=8, lr=1e-3,
     m_net = net.forward(features(S, img, c, illum))[0]
     t_net = time.time() - t
     m_rtm = np.float32(c) * img
+    t = time.time()  # the fallback check (two Born passes); the gathers still hold dobs
+    _, chosen, r_net, r_rtm, _ = guard(op, img, m_net)
+    hip_sync()
+    t_guard = time.time() - t
     t = time.time()
     iters = _Iterates()
     m_cg, ch = lsrtm.cgls(op, dobs, cg_iters, callback=iters)
     t_cg = time.time() - t
     k5, t5 = iters.first_within(m_cg)
-    t = time.time()  # the estimate with its fallback check (network + two Born passes)
-    _, chosen, r_net, r_rtm = deblur(ctx, net, S, img, c, illum, dobs)
-    hip_sync()
-    t_guard = time.time() - t
 
     res.update(
         t_image_s=t_img, t_net_s=t_net, t_cg_s=t_cg,
@@ -385,21 +321,256 @@ def run(S=None, npairs=128, panels=32, epochs=600, batch=8, lr=1e-3,
         rel_net_vs_true=rel(m_net, m_true), rel_rtm_vs_true=rel(m_rtm, m_true),
         rel_cg_vs_true=rel(m_cg, m_true),
         resid_rtm=r_rtm, resid_net=r_net, resid_cg=born_residual(ctx, m_cg, dobs),
-        chosen=chosen, t_guarded_s=t_guard, speedup_guarded_vs_cg=t_cg / (t_img + t_guard),
+        chosen=chosen, t_guard_s=t_guard, speedup_guarded_vs_cg=t_cg / (t_img + t_net + t_guard),
         cg_rr=[h["rr"] / ch[0]["rr"] for h in ch],
         cg_iters_to_5pct=k5, cg_time_to_5pct_s=t5,
         net_within_5pct=bool(rel(m_net, m_cg) <= 0.05))
     log(f"[deblur] image {t_img:.2f}s + net {t_net:.3f}s vs CGLS({cg_iters}) {t_cg:.2f}s; "
         f"rel L2 to CGLS: net {res['rel_net_vs_cg']:.3f}, scaled RTM {res['rel_rtm_vs_cg']:.3f}; "
-        f"data residual: RTM {res['resid_rtm']:.3f}, net {res['resid_net']:.3f}, "
-        f"CGLS {res['resid_cg']:.3f}; returned: {chosen} ({t_img + t_guard:.2f}s with the check)")
+        f"data residual: best-scaled RTM {res['resid_rtm']:.3f}, net {res['resid_net']:.3f}, "
+        f"CGLS {res['resid_cg']:.3f}; returned: {chosen} (check {t_guard:.2f}s)")
     return res, dict(m_true=m_true, m_rtm=m_rtm, m_net=m_net, m_cg=m_cg)
+
+
+# ---------------------------------------------------------------------------
+# The random-reflectivity configuration: 512 training and 32 held-out models
+
+CFG5 = dict(mod_nz=256, mod_nx=1024, h=10.0, tao=0.001, f0=25.0, nt=3000, nshot=64, nfdmax=4,
+            n2=30, vlo=1500.0, vhi=4000.0, src_depth=10.0)
+CFG5_CHANNELS = (3,) + (32,) * 5 + (1,)  # six 3x3 layers of 32 channels, residual on channel 0
+CFG5_TRAIN_SEED, CFG5_EVAL_SEED = 1000, 5000  # training models 1000.., held-out 5000..
+
+
+def cfg5_survey(nt=None, nshot=None):
+    """Geometry and coefficient table shared by every model: nshot surface
+    sources 10 m deep, evenly spaced, a receiver at every surface point; one
+    LS coefficient table over every 1 m/s bin of [vlo, vhi], so any
+    background's cells find their row."""
+    P = dict(CFG5)
+    if nt:
+        P["nt"] = nt
+    if nshot:
+        P["nshot"] = nshot
+    mod_nz, mod_nx, h, tao, n2 = P["mod_nz"], P["mod_nx"], P["h"], P["tao"], P["n2"]
+    cfg = host.make_config(nfdmax=P["nfdmax"], nfdmin=2, n2=n2, f0=P["f0"], fmax=2 * P["f0"] + 1,
+                           df=1.0, nthita=1000, eps=1e-5, dv=1.0, ilste=0, ifv=0, whitecoe=1e-4,
+                           hz=h, tao=tao, inorm=1, icompen=1, h=h, tao1=tao, mod_nz=mod_nz,
+                           mod_nx=mod_nx, nt1=P["nt"], s_l1=1, s_z1=1, n=mod_nx, ds=1,
+                           r_x1=mod_nx // 2, nrec=P["nshot"], dr=1)
+    d = host.derive(cfg)
+    vmin, vmax, nvel = host.vrange(np.array([P["vlo"], P["vhi"]], np.float32), 1.0)
+    need = np.ones(nvel, bool)
+    _, index, c = host.design_ls(1000, P["nfdmax"], 2, nvel, tao, h, 1.0, 1e-5, cfg.fmax, vmin,
+                                 1.0, need, d.hzx, nthreads=16)
+    cols = np.linspace(0, mod_nx - 1, P["nshot"] + 2)[1:-1].round().astype(int)
+    return dict(P=P, cfg=cfg, d=d, c=c, index=index, nvel=nvel, vmin=vmin, vmax=vmax,
+                wav=host.wavelet_table(d.nt, tao, P["f0"]), w=host.blend_weights(n2),
+                r_u=host.source_row(P["src_depth"], h, n2),
+                r_x=[int(d.s_l + cx) for cx in cols])
+
+
+def cfg5_context(S, panels):
+    """Context of `panels` groups of the survey's shots; group p images
+    columns [p*mod_nx, (p+1)*mod_nx) of a wide reflectivity and runs on its
+    own background window (cfg5_backgrounds)."""
+    P, d, cfg = S["P"], S["d"], S["cfg"]
+    ns = P["nshot"]
+    ctx = hip.Context(nz=d.nz, nx=d.nx, n2=cfg.n2, nfdmax=cfg.nfdmax, nt=d.nt, nt2=d.nt2,
+                      ilste=0, icompen=1, s_l=d.s_l, s_r=d.s_r, s_z=d.s_z, ds=1, n=cfg.n,
+                      tao2=d.tao2, h2=d.h2, taoh=d.taoh, taoh2=d.taoh2, hzx2_1=d.hzx2_1,
+                      max_shots=ns * panels, record=1, stack_nx=P["mod_nx"] * panels)
+    flat = np.full((d.nz, d.nx), S["vmin"], np.float32)
+    ctx.set_model(flat, flat * 0, S["w"], S["c"], S["index"], S["nvel"], S["vmin"], 1.0)
+    ctx.set_wavelet(S["wav"])
+    ctx.set_shot_geometry([dict(nx=d.nx, r_u=S["r_u"], r_x=S["r_x"][s], s_l=d.s_l, s_r=d.s_r,
+                                n=cfg.n, x_img=p * P["mod_nx"])
+                           for p in range(panels) for s in range(ns)])
+    ctx.panels = panels
+    return ctx
+
+
+def cfg5_backgrounds(ctx, S, raws):
+    """Background window of each group (raws: column-major [x][z] models)."""
+    P = S["P"]
+    for p, raw in enumerate(raws):
+        v, r1 = host.pad_model(raw, P["mod_nz"], P["mod_nx"], P["n2"], 0, P["tao"], P["h"])
+        for s in range(P["nshot"]):
+            ctx.set_shot_model(p * P["nshot"] + s, v, r1)
+
+
+def cfg5_models(seeds, panels):
+    """Background and reflectivity of each seed, padded to `panels` with
+    copies of the last background and zero reflectivity."""
+    bg, ms = [], []
+    for s in seeds:
+        v, m = synth.reflectors(int(s), CFG5["mod_nz"], CFG5["mod_nx"], CFG5["h"], CFG5["vlo"],
+                                CFG5["vhi"])
+        bg.append(v)
+        ms.append(m)
+    while len(bg) < panels:
+        bg.append(bg[-1])
+        ms.append(np.zeros_like(ms[-1]))
+    return bg, np.stack(ms)
+
+
+def cfg5_features(img, c, bg):
+    """Network input [n][3][z][x]: c * image, background velocity scaled to
+    [0, 1] over [vlo, vhi], depth in [0, 1]."""
+    img = np.asarray(img, np.float32)
+    n, mnz, mnx = img.shape
+    x = np.empty((n, 3, mnz, mnx), np.float32)
+    x[:, 0] = np.float32(c) * img
+    for i in range(n):
+        x[i, 1] = (bg[i].T - CFG5["vlo"]) / (CFG5["vhi"] - CFG5["vlo"])
+    x[:, 2] = np.linspace(0.0, 1.0, mnz, dtype=np.float32)[:, None]
+    return x
+
+
+def cfg5_pairs(S, seeds, panels=8):
+    """Migrated images H m_i, reflectivities m_i and backgrounds of the seeds,
+    `panels` models per launch."""
+    ctx = cfg5_context(S, panels)
+    X, M, B = [], [], []
+    t0 = time.time()
+    for b in range(0, len(seeds), panels):
+        chunk = list(seeds[b:b + panels])
+        bg, m = cfg5_models(chunk, panels)
+        cfg5_backgrounds(ctx, S, bg)
+        x = normal_images(ctx, m)
+        k = len(chunk)
+        X.append(x[:k])
+        M.append(m[:k])
+        B.extend(bg[:k])
+        log(f"[cfg5] pairs {b + k} of {len(seeds)} ({time.time() - t0:.0f}s)")
+    ctx.close()
+    return np.concatenate(X), np.concatenate(M), B
+
+
+def cfg5_train(X, M, B, c, net=None, epochs=800, patches_per_epoch=2048, patch=64, batch=256,
+               lr=1e-3, seed=0):
+    """Train (or continue training) the network on random patches of the
+    pairs; returns the network and its per-epoch losses."""
+    XF = cfg5_features(X, c, B)
+    if net is None:
+        net = cnn.Deblur(list(CFG5_CHANNELS), batch * patch * patch, batch * patch)
+        net.init_params(seed=seed)
+    rng = np.random.default_rng(seed)
+    hist = []
+    for ep in range(epochs):  # fresh random crops each epoch
+        xp, mp = patches(XF, M, patch, patches_per_epoch, rng)
+        hist.extend(net.fit(xp, mp, epochs=1, batch=batch, lr=lr, seed=seed + ep))
+        if (ep + 1) % 100 == 0:
+            log(f"[cfg5] epoch {ep + 1} of {epochs}: loss {np.mean(hist[-100:]):.4e}")
+    return net, hist
+
+
+def cfg5_load(path):
+    """A network saved by the training stages and its input scale c."""
+    net = cnn.Deblur.load(path, 256 * 64 * 64, 256 * 64)
+    with np.load(path, allow_pickle=False) as f:
+        c = float(f["scale"])
+    return net, c
+
+
+def cfg5_evaluate(S, net, c, neval=32, panels=8, cg_iters=10, seed0=CFG5_EVAL_SEED):
+    """Held-out models: Born data of each, then (timed per launch of `panels`
+    models) the migrated image, the network, the fallback check and
+    cg_iters CGLS iterations; per-model errors and misfits."""
+    ctx = cfg5_context(S, panels)
+    P = S["P"]
+    mnz, mnx = P["mod_nz"], P["mod_nx"]
+    per, times = [], dict(image=0.0, net=0.0, guard=0.0, cg=0.0)
+    for b in range(0, neval, panels):
+        seeds = list(range(seed0 + b, seed0 + min(neval, b + panels)))
+        k = len(seeds)
+        bg, mt = cfg5_models(seeds, panels)
+        cfg5_backgrounds(ctx, S, bg)
+        op = lsrtm.BornOperator(ctx, groups=panels)
+        op.forward(mt)  # observed data: Born data of the true reflectivities
+        ctx.gathers_from_records()
+        ctx.sync()
+        t = time.time()  # the migrated image: forward pass with strips + adjoint pass
+        ctx.forward_begin(True)
+        ctx.forward_advance(ctx.p.nt)
+        ctx.adjoint_begin()
+        ctx.adjoint_advance(ctx.p.nt)
+        g = ctx.adjoint_gradient()
+        img = np.ascontiguousarray(g.reshape(mnz, panels, mnx).transpose(1, 0, 2))
+        times["image"] += time.time() - t
+        t = time.time()
+        xe = cfg5_features(img, c, bg)
+        m_net = np.stack([net.forward(xe[i:i + 1])[0] for i in range(panels)])
+        times["net"] += time.time() - t
+        t = time.time()
+        m_out, choice, r_net, r_rtm, alpha = guard(op, img, m_net)
+        ctx.sync()
+        times["guard"] += time.time() - t
+        t = time.time()
+        iters = _Iterates()
+        m_cg, hist = lsrtm.cgls(op, None, cg_iters, callback=iters)
+        times["cg"] += time.time() - t
+        reach = iters.first_within(m_cg, axis_models=True)
+        resid_cg = np.sqrt(np.maximum(hist[-1]["rr"], 0.0) / hist[0]["rr"])
+        m_rtm = np.float32(c) * img
+        for i in range(k):
+            per.append(dict(seed=seeds[i], rel_net_vs_cg=rel(m_net[i], m_cg[i]),
+                            rel_rtm_vs_cg=rel(m_rtm[i], m_cg[i]),
+                            rel_out_vs_cg=rel(m_out[i], m_cg[i]),
+                            rel_net_vs_true=rel(m_net[i], mt[i]),
+                            rel_rtm_vs_true=rel(m_rtm[i], mt[i]),
+                            rel_cg_vs_true=rel(m_cg[i], mt[i]),
+                            resid_net=float(r_net[i]), resid_rtm=float(r_rtm[i]),
+                            resid_cg=float(resid_cg[i]), choice=str(choice[i]),
+                            cg_iters_to_5pct=reach[i][0], cg_s_to_5pct=reach[i][1]))
+        log(f"[cfg5] held-out {b + k} of {neval}: rel to CGLS net "
+            f"{np.mean([q['rel_net_vs_cg'] for q in per[-k:]]):.3f}, scaled image "
+            f"{np.mean([q['rel_rtm_vs_cg'] for q in per[-k:]]):.3f}")
+    ctx.close()
+    nb = (neval + panels - 1) // panels
+    res = {key: float(np.mean([q[key] for q in per])) for key in per[0]
+           if key not in ("seed", "choice", "cg_iters_to_5pct", "cg_s_to_5pct")}
+    res.update(neval=neval, panels=panels, cg_iters=cg_iters, launches=nb,
+               chose_net=sum(q["choice"] == "net" for q in per),
+               cg_iters_to_5pct=[q["cg_iters_to_5pct"] for q in per],
+               s_per_launch={k_: v / nb for k_, v in times.items()}, per_model=per)
+    return res
+
+
+def cfg5_stage(stage, out, load="", nstage=2, ntrain=512, epochs=800, panels=8, seed=0):
+    """Training stage `stage` of `nstage`: the pairs of training models
+    [stage*ntrain/nstage, (stage+1)*ntrain/nstage), then training from scratch
+    (stage 0) or from the network saved at `load`; writes the network and its
+    input scale to `out`.  Stages split the 512 pair passes over separate runs
+    of bounded length; only the network travels between them."""
+    S = cfg5_survey()
+    per = ntrain // nstage
+    seeds = range(CFG5_TRAIN_SEED + stage * per, CFG5_TRAIN_SEED + (stage + 1) * per)
+    t = time.time()
+    X, M, B = cfg5_pairs(S, seeds, panels)
+    t_pairs = time.time() - t
+    if load:
+        net, c = cfg5_load(load)
+    else:
+        c = float(np.sum(X.astype(np.float64) * M) / np.sum(X.astype(np.float64) ** 2))
+        net = None
+    t = time.time()
+    net, hist = cfg5_train(X, M, B, c, net=net, epochs=epochs, seed=seed + 7919 * stage)
+    t_train = time.time() - t
+    loss_identity = float(np.mean((np.float32(c) * X - M) ** 2))
+    net.save(out, scale=np.float64(c), stage=np.int32(stage),
+             loss=np.asarray(hist, np.float64))
+    res = dict(stage=stage, models=[seeds[0], seeds[-1]], pairs_s=t_pairs, train_s=t_train,
+               epochs=epochs, scale=c, loss_identity=loss_identity,
+               loss_first=float(np.mean(hist[:20])), loss_last=float(np.mean(hist[-20:])))
+    log(f"[cfg5] stage {stage}: pairs {t_pairs:.0f}s, training {t_train:.0f}s, loss "
+        f"{res['loss_first']:.4e} -> {res['loss_last']:.4e} (scaled image {loss_identity:.4e})")
+    return res
 
 
 def main():
     import argparse
     import json
-    ap = argparse.ArgumentParser(description="deblurring network vs CGLS on the faulted model")
+    ap = argparse.ArgumentParser(description="deblurring network vs CGLS")
     ap.add_argument("--npairs", type=int, default=128)
     ap.add_argument("--panels", type=int, default=32)
     ap.add_argument("--epochs", type=int, default=600)
@@ -411,19 +582,15 @@ def main():
     ap.add_argument("--target", default="cg", choices=["cg", "true"],
                     help="train toward the CGLS image of each pair or its reflectivity")
     ap.add_argument("--save", default="", help="write the four images to this .npz")
-    ap.add_argument("--models", action="store_true",
-                    help="many random models (1024 x 256, 64 shots, 25 Hz, 3000 steps; 6 x 32-"
-                         "channel layers on 64 x 64 patches, batch 256), held-out evaluation")
-    ap.add_argument("--ntrain", type=int, default=96)
     ap.add_argument("--save-net", default="", help="write the trained network to this .npz")
-    ap.add_argument("--neval", type=int, default=8)
+    ap.add_argument("--cfg5-stage", type=int, default=-1,
+                    help="random-reflectivity configuration: run training stage 0 or 1 "
+                         "(256 training models each) and write the network to --save-net")
+    ap.add_argument("--load-net", default="", help="stage > 0: continue from this network")
     a = ap.parse_args()
-    if a.models:
-        S = build_survey(mod_nz=256, mod_nx=1024, h=10.0, f0=25.0, nt=3000, nshot=64, nfdmax=4,
-                         n2=30)
-        res = run_models(S, ntrain=a.ntrain, neval=a.neval, epochs=a.epochs if a.epochs != 600
-                         else 100, target=a.target if "--target" in __import__("sys").argv
-                         else "true", save_net=a.save_net)
+    if a.cfg5_stage >= 0:
+        res = cfg5_stage(a.cfg5_stage, a.save_net, load=a.load_net,
+                         epochs=a.epochs if a.epochs != 600 else 800)
         print(json.dumps(res))
         return
     S = build_survey(nshot=a.shots, nt=a.nt)

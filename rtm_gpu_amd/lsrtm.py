@@ -50,7 +50,9 @@ class BornOperator:
         self.per = ctx.nshot // groups
         self._background = False  # a forward / Born pass has left its end state and strips
 
-    def _dots(self, x, y):
+    def dots(self, x, y):
+        """<x, y> of two data-space vectors (hip.DATA_GATHERS / DATA_RECORDS),
+        per group (float64; an array when groups > 1)."""
         if self.groups == 1:
             return self.ctx.data_dot(x, y)
         return np.array([self.ctx.data_dot(x, y, g * self.per, self.per)
@@ -61,7 +63,7 @@ class BornOperator:
         already hold it); returns <d, d>."""
         if d is not None:
             self.ctx.set_gathers(d)
-        return self._dots(hip.DATA_GATHERS, hip.DATA_GATHERS)
+        return self.dots(hip.DATA_GATHERS, hip.DATA_GATHERS)
 
     def forward(self, p):
         """q = L p into the records; returns <q, q>."""
@@ -74,7 +76,7 @@ class BornOperator:
         ctx.born_begin()
         ctx.born_advance(ctx.p.nt)
         self._background = True
-        return self._dots(hip.DATA_RECORDS, hip.DATA_RECORDS)
+        return self.dots(hip.DATA_RECORDS, hip.DATA_RECORDS)
 
     def update_residual(self, alpha):
         """r -= alpha q; returns <r, r>."""
@@ -83,7 +85,7 @@ class BornOperator:
         else:
             for g in range(self.groups):
                 self.ctx.data_axpy(-np.float32(alpha[g]), g * self.per, self.per)
-        return self._dots(hip.DATA_GATHERS, hip.DATA_GATHERS)
+        return self.dots(hip.DATA_GATHERS, hip.DATA_GATHERS)
 
     def adjoint(self):
         """s = L^T r (the stacked gradient)."""

@@ -6,6 +6,8 @@ the on-disk layout the reference reads (GPU_velocity_real.cpp:12-18).
   faulted_rvsp  351 x 1701, 8 dipping layers, 3 normal faults, a 1200 m x 600 m
                 high-velocity body                          (reverse VSP, 12.5 m)
   grf_4001      801 x 4001 Gaussian-random-field background + 6 reflectors (10 m)
+  reflectors    256 x 1024 gradient + Gaussian-random-field background and
+                10-40 dipping planar reflectors       (learned deblurring, 10 m)
 """
 import numpy as np
 
@@ -80,3 +82,50 @@ def smooth(raw_colmajor, sigma_cells):
     from scipy.ndimage import gaussian_filter
     return np.ascontiguousarray(gaussian_filter(raw_colmajor.astype(np.float64), sigma_cells,
                                                 mode="nearest"), np.float32)
+
+
+def _grf(rng, mod_nz, mod_nx, h, corr):
+    """Zero-mean, unit-variance Gaussian random field [z][x] with a Gaussian
+    spectrum of correlation length corr metres."""
+    kx = np.fft.rfftfreq(mod_nx, d=h)
+    kz = np.fft.fftfreq(mod_nz, d=h)
+    k2 = kz[:, None] ** 2 + kx[None, :] ** 2
+    spec = np.exp(-0.5 * k2 * (2 * np.pi * corr / 2.0) ** 2)
+    noise = rng.normal(0, 1, (mod_nz, kx.size)) + 1j * rng.normal(0, 1, (mod_nz, kx.size))
+    f = np.fft.irfft2(noise * spec, s=(mod_nz, mod_nx))
+    return (f - f.mean()) / f.std()
+
+
+def reflectors(seed, mod_nz=256, mod_nx=1024, h=10.0, vtop=1500.0, vbot=4000.0, corr=400.0,
+               grf_std=200.0, nmin=10, nmax=40, dipmax=30.0, rmax=0.1):
+    """A random reflectivity model for the deblurring network.
+
+    Background: the linear gradient vtop (surface) .. vbot (bottom row) plus a
+    Gaussian random field of standard deviation grf_std m/s and correlation
+    length corr m, clipped to [vtop, vbot].  Reflectivity: nmin..nmax planar
+    interfaces across the model, each through a seeded point (x0 anywhere, z0
+    in the upper 95% of the depth range) with a dip uniform in +-dipmax
+    degrees and a reflection strength uniform in +-rmax, drawn one cell thick
+    (the strength split linearly between the two rows around the interface
+    depth in each column); crossing interfaces add.
+
+    Returns (background column-major [x][z], reflectivity [z][x]), float32."""
+    rng = np.random.default_rng(seed)
+    z = np.arange(mod_nz, dtype=np.float64)
+    grad = vtop + (vbot - vtop) * z / (mod_nz - 1)
+    v = np.clip(grad[:, None] + grf_std * _grf(rng, mod_nz, mod_nx, h, corr), vtop, vbot)
+    m = np.zeros((mod_nz, mod_nx), np.float64)
+    x = np.arange(mod_nx, dtype=np.float64)
+    cols = np.arange(mod_nx)
+    for _ in range(int(rng.integers(nmin, nmax + 1))):
+        dip = np.deg2rad(rng.uniform(-dipmax, dipmax))
+        x0 = rng.uniform(0.0, mod_nx - 1.0)
+        z0 = rng.uniform(0.05, 0.95) * (mod_nz - 1)
+        r = rng.uniform(-rmax, rmax)
+        zi = z0 + np.tan(dip) * (x - x0)
+        k = np.floor(zi).astype(np.int64)
+        f = zi - k
+        ok = (k >= 0) & (k < mod_nz - 1)
+        np.add.at(m, (k[ok], cols[ok]), r * (1.0 - f[ok]))
+        np.add.at(m, (k[ok] + 1, cols[ok]), r * f[ok])
+    return np.ascontiguousarray(v.T, np.float32), m.astype(np.float32)

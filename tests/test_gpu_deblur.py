@@ -60,38 +60,74 @@ def test_grouped_cgls_on_panels_equals_single_runs():
 
 
 def test_plain_migration_is_the_fallback():
-    """deblur() returns the network's estimate only when its Born re-modelled
-    data fit the observations better than those of the scaled migrated image."""
-    from rtm_gpu_amd import cnn, deblur
+    """guard() returns the network's estimate only when its Born re-modelled
+    data fit the observations better than both zero and the best multiple
+    alpha = <L img, d> / ||L img||^2 of the migrated image; otherwise that
+    multiple, whose misfit is at most 1."""
+    from rtm_gpu_amd import cnn, deblur, lsrtm
     S = _survey()
     ctx = deblur.context(S)
+    op = lsrtm.BornOperator(ctx)
     m_true = S["m_true"]
     ctx.set_reflectivity(m_true)
     dobs = ctx.born()
     ctx.forward_begin(True)
     ctx.forward_advance(ctx.p.nt)
-    img = ctx.adjoint(dobs)
+    img = ctx.adjoint(dobs)  # leaves the gathers holding dobs
+    ctx.set_reflectivity(img)
+    limg = ctx.born().astype(np.float64)
+    d64 = dobs.astype(np.float64)
+    alpha = float(np.sum(limg * d64) / np.sum(limg * limg))
+    want_rtm = float(np.linalg.norm(alpha * limg - d64) / np.linalg.norm(d64))
+    ctx.set_gathers(dobs)
+
+    m, which, r_net, r_rtm, a = deblur.guard(op, img, m_true)  # the exact reflectivity
+    assert which == "net" and r_net <= 1e-6 and np.array_equal(m, m_true)
+    assert a == pytest.approx(alpha, rel=1e-6) and r_rtm == pytest.approx(want_rtm, rel=1e-5)
+    assert 0.0 < r_rtm < 1.0
+
+    # the negated reflectivity fits worse than zero (misfit 2): rejected
+    m, which, r_net, _, _ = deblur.guard(op, img, -m_true)
+    assert which == "rtm" and r_net == pytest.approx(2.0, rel=1e-5)
+    assert np.array_equal(m, np.float32(alpha) * img)
+
+    # an image-domain multiple no better than the best one: rejected
     c = float(np.sum(img.astype(np.float64) * m_true) / np.sum(img.astype(np.float64) ** 2))
+    zero = cnn.Deblur([3, 8, 1], img.size, img.shape[0])
+    zero.init_params(seed=1)  # last layer zero: the output is the scaled image c * img
     illum = deblur.illumination(S)
-    mnz, mnx = img.shape
+    ctx.set_gathers(dobs)
+    m_zero = zero.forward(deblur.features(S, img, c, illum))[0]
+    assert np.array_equal(m_zero, np.float32(c) * img)
+    m, which, r_net, r_rtm, _ = deblur.guard(op, img, m_zero)
+    assert which == "rtm" and r_rtm <= r_net + 1e-9
 
-    class Exact:  # stands in for a network that recovers the reflectivity
-        def forward(self, x):
-            return np.asarray(m_true, np.float32)[None]
+    # a network whose output is not finite is rejected without a Born pass of it
+    bad = np.full_like(img, np.nan)
+    m, which, r_net, r_rtm, _ = deblur.guard(op, img, bad)
+    assert which == "rtm" and r_net == np.inf and np.isfinite(m).all()
 
-    m, which, r_net, r_rtm = deblur.deblur(ctx, Exact(), S, img, c, illum, dobs)
-    assert which == "net" and r_net == 0.0 < r_rtm and np.array_equal(m, m_true)
 
-    zero = cnn.Deblur([3, 8, 1], mnz * mnx, mnz)
-    zero.init_params(seed=1)  # last layer zero: the output is the scaled image
-    m, which, r_net, r_rtm = deblur.deblur(ctx, zero, S, img, c, illum, dobs)
-    assert which == "rtm" and r_net == r_rtm
-    assert np.array_equal(m, np.float32(c) * img)
-
-    bad = cnn.Deblur([3, 8, 1], mnz * mnx, mnz)
-    p = bad.init_params(seed=2, last_zero=False)
-    bad.set_params(p * 50)
-    m, which, r_net, r_rtm = deblur.deblur(ctx, bad, S, img, c, illum, dobs)
-    print(f"misfit: scaled image {r_rtm:.3f}, inflated network {r_net:.3f}")
-    assert which == "rtm" and not r_net < r_rtm
-    assert np.array_equal(m, np.float32(c) * img)
+def test_cfg5_pipeline_full_size():
+    """The random-reflectivity configuration at its full size (1024 x 256,
+    64 shots, 3000 steps) for two models in one launch: pairs are finite and
+    not zero, and the held-out evaluation orders the data misfits as the
+    method fixes them (CGLS(2) below the best-scaled image below zero), with
+    the fallback taken for a network that only scales the image."""
+    from rtm_gpu_amd import cnn, deblur
+    S = deblur.cfg5_survey()
+    print("cfg5 full size: two models, 128 shots, 3000 steps", flush=True)
+    X, M, B = deblur.cfg5_pairs(S, [deblur.CFG5_TRAIN_SEED, deblur.CFG5_TRAIN_SEED + 1], panels=2)
+    assert X.shape == (2, 256, 1024) and np.isfinite(X).all()
+    assert all(np.abs(x).max() > 0 for x in X)
+    c = float(np.sum(X.astype(np.float64) * M) / np.sum(X.astype(np.float64) ** 2))
+    assert c > 0
+    net = cnn.Deblur(list(deblur.CFG5_CHANNELS), 256 * 1024, 256)
+    net.init_params(seed=0)  # zero last layer: output c * image
+    res = deblur.cfg5_evaluate(S, net, c, neval=2, panels=2, cg_iters=2)
+    print({k: v for k, v in res.items() if k != "per_model"})
+    for q in res["per_model"]:
+        assert q["choice"] == "rtm"
+        assert q["resid_cg"] < q["resid_rtm"] < 1.0
+        assert q["rel_cg_vs_true"] < 1.0
+        assert np.isfinite(q["rel_net_vs_cg"])
