@@ -131,7 +131,10 @@ int rtm_set_shot_geometry(rtm_ctx* ctx, int nshot, const rtm_shot_geom* geom);
  * builds them for that window.  Call after rtm_set_model (which fixes the
  * coefficient table, w, vmin and dv) and rtm_set_shot_geometry; once any shot
  * has a window, every shot of the batch must have one.  rtm_set_model returns
- * the context to one model for all shots. */
+ * the context to one model for all shots.  Shots whose windows are equal
+ * (same width and bytes, compared through two 64-bit fingerprints) share one
+ * stored window, so a batch of many shots over a few models reads each
+ * model's arrays once per step from cache rather than once per shot. */
 int rtm_set_shot_model(rtm_ctx* ctx, int shot, const float* v, const float* r1);
 /* Observed gathers of the batch, seis[shot][n][nt] as read at kernel.cu:827-856.
  * Uploaded once per batch (the reference copies one time slice per step,

@@ -56,10 +56,14 @@ struct rtm_ctx {
   int gw = 0;                  // columns of the stacked image
   int4* d_tiles = nullptr;     // frame (tile, shot) records of the batch
   int ntiles = 0, tile_cap = 0;
-  // per-shot model windows of a ragged batch (allocated on first use)
+  // model windows of a ragged batch (allocated on first use): slot w holds
+  // one distinct window; shots whose windows are equal share a slot
   float *d_mv = nullptr, *d_mr1 = nullptr, *d_mpm = nullptr;
   unsigned* d_mko = nullptr;
   std::vector<char> shot_model;
+  std::vector<int> win_ref;                 // shots using each slot
+  std::vector<int> win_nx;                  // width of each slot's window
+  std::vector<unsigned long long> win_fp;   // two 64-bit content fingerprints per slot
   float tab_vmin = 0.f, tab_dv = 1.f;  // bin parameters of the current table
   int tab_nrow = 1, tab_cs = 4;
   bool tab_ls = true;
@@ -762,6 +766,9 @@ int rtm_set_model(rtm_ctx* c, const float* v, const float* r1, const float* w, c
   m.pm = c->d_pm; m.ko = c->d_ko;
   m.cs = CS; m.nrow = nrow;
   c->shot_model.assign(c->p.max_shots, 0);
+  c->win_ref.assign(c->p.max_shots, 0);
+  c->win_nx.assign(c->p.max_shots, 0);
+  c->win_fp.assign(2 * (size_t)c->p.max_shots, 0);
   m.hzx2_1 = c->p.hzx2_1; m.tao2 = c->p.tao2; m.h2 = c->p.h2;
   m.taoh = c->p.taoh; m.taoh2 = c->p.taoh2;
   c->nfd = nfd;
@@ -787,7 +794,7 @@ int rtm_set_shots(rtm_ctx* c, int nshot, const int* r_u, const int* r_x) {
   for (int i = 0; i < nshot; ++i) {
     if (r_u[i] < 0 || r_u[i] >= g.nz || r_x[i] < 0 || r_x[i] >= g.nx)
       return fail(c, RTM_E_ARG, "shot %d source (%d,%d) outside grid", i, r_u[i], r_x[i]);
-    geo[i] = ShotGeo{g.nx, g.mnx, c->p.s_l, c->p.s_r, c->p.n, r_u[i], r_x[i], 0};
+    geo[i] = ShotGeo{g.nx, g.mnx, c->p.s_l, c->p.s_r, c->p.n, r_u[i], r_x[i], 0, 0};
   }
   if (c->m.mstride != 0) return fail(c, RTM_E_STATE, "per-shot model windows need rtm_set_shot_geometry");
   c->geo = geo;
@@ -810,14 +817,35 @@ int rtm_set_shot_geometry(rtm_ctx* c, int nshot, const rtm_shot_geom* sg) {
       return fail(c, RTM_E_ARG, "shot %d receivers s_l=%d s_r=%d n=%d", i, q.s_l, q.s_r, q.n);
     if (q.x_img < 0 || q.x_img + mnx > c->gw)
       return fail(c, RTM_E_ARG, "shot %d image columns %d..%d outside 0..%d", i, q.x_img, q.x_img + mnx, c->gw);
-    geo[i] = ShotGeo{q.nx, mnx, q.s_l, q.s_r, q.n, q.r_u, q.r_x, q.x_img};
+    geo[i] = ShotGeo{q.nx, mnx, q.s_l, q.s_r, q.n, q.r_u, q.r_x, q.x_img, 0};
   }
   c->geo = geo;
   // Model windows were laid out for the previous batch's widths: once the
   // context holds per-shot windows, every shot of the new batch needs its
   // own again (rtm_forward_begin / rtm_born_begin check this).
   std::fill(c->shot_model.begin(), c->shot_model.end(), 0);
+  c->win_ref.assign(c->p.max_shots, 0);
+  c->win_nx.assign(c->p.max_shots, 0);
+  c->win_fp.assign(2 * (size_t)c->p.max_shots, 0);
+  c->tp_ok = false;
   return upload_geometry(c);
+}
+
+// Two independent 64-bit fingerprints of a window's bytes (width included);
+// windows whose fingerprints and widths agree share one slot.
+static void window_fingerprint(int nx, const float* v, const float* r1, size_t n,
+                               unsigned long long* f) {
+  unsigned long long a = 0xcbf29ce484222325ULL ^ (unsigned)nx, b = 0x9e3779b97f4a7c15ULL + (unsigned)nx;
+  for (const float* p : {v, r1})
+    for (size_t i = 0; i < n; ++i) {
+      unsigned w;
+      memcpy(&w, p + i, 4);
+      a = (a ^ w) * 0x100000001b3ULL;
+      b = (b + w + 0x9e3779b97f4a7c15ULL) * 0xbf58476d1ce4e5b9ULL;
+      b ^= b >> 31;
+    }
+  f[0] = a;
+  f[1] = b;
 }
 
 int rtm_set_shot_model(rtm_ctx* c, int shot, const float* v, const float* r1) {
@@ -833,15 +861,42 @@ int rtm_set_shot_model(rtm_ctx* c, int shot, const float* v, const float* r1) {
     if (rc == RTM_OK) rc = dalloc_t(c, &c->d_mko, n);
     if (rc != RTM_OK) return rc;
   }
-  const size_t o = (size_t)shot * g.fstride;
-  int rc = upload_model_window(c, c->geo[shot].nx, v, r1, c->d_mv + o, c->d_mr1 + o, c->d_mpm + o,
-                               c->d_mko + o);
-  if (rc != RTM_OK) return rc;
+  if (c->win_ref.size() != (size_t)c->p.max_shots) {
+    c->win_ref.assign(c->p.max_shots, 0);
+    c->win_nx.assign(c->p.max_shots, 0);
+    c->win_fp.assign(2 * (size_t)c->p.max_shots, 0);
+  }
+  const int nx = c->geo[shot].nx;
+  unsigned long long fp[2];
+  window_fingerprint(nx, v, r1, (size_t)g.nz * nx, fp);
+  if (c->shot_model[shot]) c->win_ref[c->geo[shot].win] -= 1;  // the shot's previous window
+  int w = -1;
+  for (int i = 0; i < c->p.max_shots && w < 0; ++i)
+    if (c->win_ref[i] > 0 && c->win_nx[i] == nx && c->win_fp[2 * i] == fp[0] && c->win_fp[2 * i + 1] == fp[1])
+      w = i;
+  if (w < 0) {  // a new window: the first slot no shot uses
+    for (int i = 0; i < c->p.max_shots && w < 0; ++i)
+      if (c->win_ref[i] == 0) w = i;
+    if (w < 0) return fail(c, RTM_E_STATE, "no free model window slot");
+    const size_t o = (size_t)w * g.fstride;
+    int rc = upload_model_window(c, nx, v, r1, c->d_mv + o, c->d_mr1 + o, c->d_mpm + o, c->d_mko + o);
+    if (rc != RTM_OK) {
+      c->shot_model[shot] = 0;
+      return rc;
+    }
+    c->win_nx[w] = nx;
+    c->win_fp[2 * w] = fp[0];
+    c->win_fp[2 * w + 1] = fp[1];
+    c->tp_ok = false;
+  }
+  c->win_ref[w] += 1;
+  c->geo[shot].win = w;
+  HIPCHK(c, hipMemcpyAsync(c->d_geo + shot, &c->geo[shot], sizeof(ShotGeo), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   Model& m = c->m;
   m.mstride = g.fstride;
   m.v = c->d_mv; m.r1 = c->d_mr1; m.pm = c->d_mpm; m.ko = c->d_mko;
   c->shot_model[shot] = 1;
-  c->tp_ok = false;
   drop_graph(c);
   return RTM_OK;
 }
@@ -1017,7 +1072,9 @@ int rtm_adjoint_begin(rtm_ctx* c) {
   if (!c->data_ok) return fail(c, RTM_E_STATE, "rtm_set_gathers first (residuals)");
   if (c->p.ilste == 0 && !c->tp_ok) {  // neighbour-tap planes of the current model windows
     const Grid& g = c->g;
-    const int nwin = c->m.mstride ? c->nshot : 1;
+    int nwin = 1;  // slots 0 .. nwin-1 (one model: slot 0)
+    if (c->m.mstride)
+      for (int s = 0; s < c->nshot; ++s) nwin = std::max(nwin, c->geo[s].win + 1);
     const size_t per = (size_t)4 * c->nfd * g.fstride;
     if (per * sizeof(float) >= (1ull << 31))
       return fail(c, RTM_E_ARG, "grid too large for the transposed adjoint stencil");
@@ -1030,8 +1087,9 @@ int rtm_adjoint_begin(rtm_ctx* c) {
       c->tp_cap = per * nwin;
     }
     for (int w = 0; w < nwin; ++w)
-      HIPCHK(c, launch_tap_planes(g, c->m.ko + (size_t)w * c->m.mstride, c->m.ctab, c->nfd,
-                                  c->d_tp + (size_t)w * per, c->stream));
+      if (!c->m.mstride || c->win_ref[w] > 0)
+        HIPCHK(c, launch_tap_planes(g, c->m.ko + (size_t)w * c->m.mstride, c->m.ctab, c->nfd,
+                                    c->d_tp + (size_t)w * per, c->stream));
     c->m.tp = c->d_tp;
     c->m.tpstride = c->m.mstride ? (long long)per : 0;
     c->tp_ok = true;

@@ -20,7 +20,7 @@ struct Grid {
 // Model operands.  ctab rows are [c0' as double | c_1 .. c_NFD | zero pad],
 // c0' = (1.0 + hzx2_1) * c_0 in double (the promotion of kernel.cu:58).
 struct Model {
-  long long mstride;   // 0: one model for all shots; fstride: one model window per shot
+  long long mstride;   // 0: one model for all shots; fstride: windows ShotGeo::win apart
   const float* v;
   const float* r1;
   const float* pm;     // v*v*tao2*h2 per cell (FP32, left to right), padded layout
@@ -52,6 +52,7 @@ struct ShotGeo {
   int s_l, s_r, n;     // receivers on row s_z, spacing ds (common)
   int r_u, r_x;        // source
   int x_img;           // image column of interior column 0
+  int win;             // model window of the shot (Model::mstride apart; 0 with one model)
 };
 
 // Strip storage: fp32, or fp16/bf16 with a per-(shot, slot) scale.

@@ -70,6 +70,23 @@ __device__ __forceinline__ void bst4(Rsrc r, unsigned vo, unsigned so, float a, 
   const u32x4 v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c), __float_as_uint(d)};
   __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(vo + so), 0, 0);
 }
+// Streaming variants for data touched once per time step (u(k+1), u0, the
+// image and illumination sums): the non-temporal policy keeps them from
+// displacing what the shots of a launch share in the caches (the model's P
+// and coefficient offsets, the coefficient table, the u1 halo rows).
+#ifndef RTM_STREAM_POLICY
+#define RTM_STREAM_POLICY 2  // cache-policy bits of the buffer instructions: nt
+#endif
+__device__ __forceinline__ float4 bld4s(Rsrc r, unsigned vo, unsigned so) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)vo, (int)so, RTM_STREAM_POLICY);
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                     __uint_as_float(v.w));
+}
+__device__ __forceinline__ void bst4s(Rsrc r, unsigned vo, unsigned so, float a, float b, float c,
+                                      float d) {
+  const u32x4 v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c), __float_as_uint(d)};
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(vo + so), 0, RTM_STREAM_POLICY);
+}
 
 template <int NFD>
 struct CoefRow {
@@ -317,13 +334,13 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   const long long so = (long long)shot * g.fstride;
   const unsigned fb = (unsigned)g.fstride * 4u;
   const Rsrc R1 = rsrc(a.u1 + so, fb), R0 = rsrc(a.u0 + so, fb), R2 = rsrc(a.u2 + so, fb);
-  const long long mo = (long long)shot * m.mstride;
+  const long long mo = (long long)sg.win * m.mstride;
   const Rsrc RP = rsrc(m.pm + mo, fb), RK = rsrc(m.ko + mo, fb);
   const Rsrc RC = rsrc(m.ctab, (unsigned)(m.nrow * m.cs) * 4u);
   const unsigned P4 = (unsigned)g.pitch * 4u;
   auto rowb = [&](int z) -> unsigned { return (unsigned)min(z, g.nz - 1) * P4; };
   Rsrc RTP;
-  if (T::adjt) RTP = rsrc(m.tp + (long long)shot * m.tpstride, (unsigned)(4 * NFD) * fb);
+  if (T::adjt) RTP = rsrc(m.tp + (long long)sg.win * m.tpstride, (unsigned)(4 * NFD) * fb);
 
   const int2 sp = make_int2(sg.r_u, sg.r_x);
   Rcv rr = a.r;  // this shot's receivers; a.r.n stays the stride of the data layout
@@ -375,7 +392,7 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   for (int d = 0; d < PD; ++d) {
     kb[d] = bld4u(RK, vcol, rowb(zs + d));
     pb[d] = bld4(RP, vcol, rowb(zs + d));
-    u0b[d] = bld4(R0, vcol, rowb(zs + d));
+    u0b[d] = bld4s(R0, vcol, rowb(zs + d));
     hvb[d] = bld4(R1, vhal, rowb(zs + d));
   }
   float4 ukb[1] = {}, i1b[1] = {}, i2b[1] = {}, sub[1] = {}, sqb[1] = {};
@@ -415,21 +432,21 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
       if (T::rcv) {
         const unsigned rzz = rowb(zz);
         if (!T::rcvi) ukb[0] = bld4(RU, vcol, rzz);  // the split step never needs u(k)
-        i1b[0] = bld4(RI1, vcol, rzz);
-        if (!T::rcvi) i2b[0] = bld4(RI2, vcol, rzz);
-        sub[0] = bld4(RSU, vcs, rzz);
-        sqb[0] = bld4(RSQ, vcs, rzz);
+        i1b[0] = bld4s(RI1, vcol, rzz);
+        if (!T::rcvi) i2b[0] = bld4s(RI2, vcol, rzz);
+        sub[0] = bld4s(RSU, vcs, rzz);
+        sqb[0] = bld4s(RSQ, vcs, rzz);
       }
       if (T::srci) {  // this row's sum u and I2, for the source-side image terms
         const unsigned rzz = rowb(zz);
-        i2b[0] = bld4(RI2, vcol, rzz);
-        sub[0] = bld4(RSU, vcs, rzz);
+        i2b[0] = bld4s(RI2, vcol, rzz);
+        sub[0] = bld4s(RSU, vcs, rzz);
       }
       // (b) operands of later rows
       q[(j + 2 * NFD + PD) % QL] = bld4(R1, vcol, rowb(zz + NFD + PD));
       kb[PD] = bld4u(RK, vcol, rowb(zz + PD));
       pb[PD] = bld4(RP, vcol, rowb(zz + PD));
-      u0b[PD] = bld4(R0, vcol, rowb(zz + PD));
+      u0b[PD] = bld4s(R0, vcol, rowb(zz + PD));
       hvb[PD] = bld4(R1, vhal, rowb(zz + PD));
       if (T::born || T::adj) {
         xab[1] = bld4(RA, vxl, rowb(zz + 1));
@@ -490,7 +507,7 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
       }
       const unsigned rowdrop = zz < zhi ? 0u : kDrop;
       const unsigned rz = rowb(zz);
-      bst4(R2, vst | rowdrop, rz, out[0], out[1], out[2], out[3]);
+      bst4s(R2, vst | rowdrop, rz, out[0], out[1], out[2], out[3]);
       if (T::rcv) {  // imaging (kernel.cu:489-517); frame columns keep their values
         const float4 ukv = ukb[0], i1 = i1b[0], i2 = i2b[0], su = sub[0], sq = sqb[0];
         float r1o[4], r2o[4], rsu[4], rsq[4];
@@ -506,12 +523,12 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
           rsu[jj] = colok[jj] ? rsu[jj] : compr(su, jj);
           rsq[jj] = colok[jj] ? rsq[jj] : compr(sq, jj);
         }
-        bst4(RI1, vst | rowdrop, rz, r1o[0], r1o[1], r1o[2], r1o[3]);
+        bst4s(RI1, vst | rowdrop, rz, r1o[0], r1o[1], r1o[2], r1o[3]);
         if (!T::rcvi) {
-          bst4(RI2, vst | rowdrop, rz, r2o[0], r2o[1], r2o[2], r2o[3]);
-          bst4(RSU, vss | rowdrop, rz, rsu[0], rsu[1], rsu[2], rsu[3]);
+          bst4s(RI2, vst | rowdrop, rz, r2o[0], r2o[1], r2o[2], r2o[3]);
+          bst4s(RSU, vss | rowdrop, rz, rsu[0], rsu[1], rsu[2], rsu[3]);
         }
-        bst4(RSQ, vss | rowdrop, rz, rsq[0], rsq[1], rsq[2], rsq[3]);
+        bst4s(RSQ, vss | rowdrop, rz, rsq[0], rsq[1], rsq[2], rsq[3]);
       }
       if (T::srci) {  // source-side image terms of step k on the u(k) just computed
         const float4 i2 = i2b[0], su = sub[0];
@@ -522,8 +539,8 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
           rsu[jj] = colok[jj] ? compr(su, jj) + u : compr(su, jj);
           r2o[jj] = colok[jj] ? __builtin_fmaf(u, u, compr(i2, jj)) : compr(i2, jj);
         }
-        bst4(RI2, vst | rowdrop, rz, r2o[0], r2o[1], r2o[2], r2o[3]);
-        bst4(RSU, vss | rowdrop, rz, rsu[0], rsu[1], rsu[2], rsu[3]);
+        bst4s(RI2, vst | rowdrop, rz, r2o[0], r2o[1], r2o[2], r2o[3]);
+        bst4s(RSU, vss | rowdrop, rz, rsu[0], rsu[1], rsu[2], rsu[3]);
       }
       if (T::adj) {  // G += D2u(k+1) * mu(k+2); mu(k+2) is this update's u0
         float go[4];
@@ -626,7 +643,7 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   const long long so = (long long)shot * g.fstride;
   const unsigned fb = (unsigned)g.fstride * 4u;
   const Rsrc R1 = rsrc(a.u1 + so, fb), R0 = rsrc(a.u0 + so, fb), R2 = rsrc(a.u2 + so, fb);
-  const long long mo = (long long)shot * m.mstride;
+  const long long mo = (long long)sg.win * m.mstride;
   const Rsrc RP = rsrc(m.pm + mo, fb), RK = rsrc(m.ko + mo, fb);
   const Rsrc RV = rsrc(m.v + mo, fb), RR = rsrc(m.r1 + mo, fb);
   const Rsrc RC = rsrc(m.ctab, (unsigned)(m.nrow * m.cs) * 4u);
@@ -670,7 +687,7 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   // reads) take the transposed stencil of the interior kernel
   const bool adjt = MODE == FR_ADJ && a.adjt != 0;
   Rsrc RTP;
-  if (MODE == FR_ADJ) RTP = rsrc(m.tp + (long long)shot * m.tpstride, adjt ? (unsigned)(4 * NFD) * fb : 0u);
+  if (MODE == FR_ADJ) RTP = rsrc(m.tp + (long long)sg.win * m.tpstride, adjt ? (unsigned)(4 * NFD) * fb : 0u);
   Rsrc RBX, RBA, RBB, RBC;
   if (MODE == FR_BORN) {
     RBX = rsrc(m.refl + (long long)shot * m.rstride, fb);
@@ -1135,7 +1152,7 @@ __global__ void k_post(Grid g, const ShotGeo* geo, const float* v, long long mst
   const int mnx = sg.mnx, nx = sg.nx;
   const float* M = i1 + (long long)shot * g.fstride + g.x0;
   const float* I = i2 + (long long)shot * g.fstride + g.x0;
-  const float* Vv = v + (long long)shot * mstride + g.x0;
+  const float* Vv = v + (long long)sg.win * mstride + g.x0;
   const long long n = (long long)g.mnz * mnx;
   const float stable = __uint_as_float(amax[shot]) * whitecoe;
   const int P = g.pitch, n2 = g.n2;
@@ -1207,7 +1224,7 @@ __global__ void k_grad_stack(Grid g, const ShotGeo* geo, int gw, const float* G,
       const int x = X - geo[s].x_img;
       if (x < 0 || x >= geo[s].mnx) continue;
       const long long pc = (long long)(z + g.n2) * g.pitch + g.x0 + g.n2 + x;
-      acc = acc + G[(long long)s * g.fstride + pc] / pm[(long long)s * mstride + pc];
+      acc = acc + G[(long long)s * g.fstride + pc] / pm[(long long)geo[s].win * mstride + pc];
     }
     grad[e] = acc;
   }

@@ -413,13 +413,15 @@ def cfg5_models(seeds, panels):
     return bg, np.stack(ms)
 
 
-def cfg5_features(img, c, bg):
-    """Network input [n][3][z][x]: c * image, background velocity scaled to
+def cfg5_features(img, c, bg, norm):
+    """Network input [n][3][z][x]: c * image / norm (norm: the RMS of the
+    training reflectivities, so that inputs and targets are of order one and
+    Adam's steps are small against them), background velocity scaled to
     [0, 1] over [vlo, vhi], depth in [0, 1]."""
     img = np.asarray(img, np.float32)
     n, mnz, mnx = img.shape
     x = np.empty((n, 3, mnz, mnx), np.float32)
-    x[:, 0] = np.float32(c) * img
+    x[:, 0] = np.float32(c / norm) * img
     for i in range(n):
         x[i, 1] = (bg[i].T - CFG5["vlo"]) / (CFG5["vhi"] - CFG5["vlo"])
     x[:, 2] = np.linspace(0.0, 1.0, mnz, dtype=np.float32)[:, None]
@@ -446,11 +448,12 @@ def cfg5_pairs(S, seeds, panels=8):
     return np.concatenate(X), np.concatenate(M), B
 
 
-def cfg5_train(X, M, B, c, net=None, epochs=800, patches_per_epoch=2048, patch=64, batch=256,
-               lr=1e-3, seed=0):
+def cfg5_train(X, M, B, c, norm, net=None, epochs=800, patches_per_epoch=2048, patch=64,
+               batch=256, lr=1e-3, seed=0):
     """Train (or continue training) the network on random patches of the
-    pairs; returns the network and its per-epoch losses."""
-    XF = cfg5_features(X, c, B)
+    pairs (targets m / norm); returns the network and its per-epoch losses."""
+    XF = cfg5_features(X, c, B, norm)
+    M = np.asarray(M, np.float32) / np.float32(norm)
     if net is None:
         net = cnn.Deblur(list(CFG5_CHANNELS), batch * patch * patch, batch * patch)
         net.init_params(seed=seed)
@@ -465,14 +468,21 @@ def cfg5_train(X, M, B, c, net=None, epochs=800, patches_per_epoch=2048, patch=6
 
 
 def cfg5_load(path):
-    """A network saved by the training stages and its input scale c."""
+    """A network saved by the training stages, its image scale c and the
+    reflectivity normalisation norm."""
     net = cnn.Deblur.load(path, 256 * 64 * 64, 256 * 64)
     with np.load(path, allow_pickle=False) as f:
-        c = float(f["scale"])
-    return net, c
+        c, norm = float(f["scale"]), float(f["norm"])
+    return net, c, norm
 
 
-def cfg5_evaluate(S, net, c, neval=32, panels=8, cg_iters=10, seed0=CFG5_EVAL_SEED):
+def cfg5_estimate(net, img, c, norm, bg):
+    """The network's reflectivity estimates of the images img [n][z][x]."""
+    xe = cfg5_features(img, c, bg, norm)
+    return np.stack([net.forward(xe[i:i + 1])[0] for i in range(len(xe))]) * np.float32(norm)
+
+
+def cfg5_evaluate(S, net, c, norm, neval=32, panels=8, cg_iters=10, seed0=CFG5_EVAL_SEED):
     """Held-out models: Born data of each, then (timed per launch of `panels`
     models) the migrated image, the network, the fallback check and
     cg_iters CGLS iterations; per-model errors and misfits."""
@@ -498,8 +508,7 @@ def cfg5_evaluate(S, net, c, neval=32, panels=8, cg_iters=10, seed0=CFG5_EVAL_SE
         img = np.ascontiguousarray(g.reshape(mnz, panels, mnx).transpose(1, 0, 2))
         times["image"] += time.time() - t
         t = time.time()
-        xe = cfg5_features(img, c, bg)
-        m_net = np.stack([net.forward(xe[i:i + 1])[0] for i in range(panels)])
+        m_net = cfg5_estimate(net, img, c, norm, bg)
         times["net"] += time.time() - t
         t = time.time()
         m_out, choice, r_net, r_rtm, alpha = guard(op, img, m_net)
@@ -549,18 +558,19 @@ def cfg5_stage(stage, out, load="", nstage=2, ntrain=512, epochs=800, panels=8, 
     X, M, B = cfg5_pairs(S, seeds, panels)
     t_pairs = time.time() - t
     if load:
-        net, c = cfg5_load(load)
+        net, c, norm = cfg5_load(load)
     else:
         c = float(np.sum(X.astype(np.float64) * M) / np.sum(X.astype(np.float64) ** 2))
+        norm = float(np.sqrt(np.mean(M.astype(np.float64) ** 2)))
         net = None
     t = time.time()
-    net, hist = cfg5_train(X, M, B, c, net=net, epochs=epochs, seed=seed + 7919 * stage)
+    net, hist = cfg5_train(X, M, B, c, norm, net=net, epochs=epochs, seed=seed + 7919 * stage)
     t_train = time.time() - t
-    loss_identity = float(np.mean((np.float32(c) * X - M) ** 2))
-    net.save(out, scale=np.float64(c), stage=np.int32(stage),
+    loss_identity = float(np.mean(((np.float32(c) * X - M) / np.float32(norm)) ** 2))
+    net.save(out, scale=np.float64(c), norm=np.float64(norm), stage=np.int32(stage),
              loss=np.asarray(hist, np.float64))
     res = dict(stage=stage, models=[seeds[0], seeds[-1]], pairs_s=t_pairs, train_s=t_train,
-               epochs=epochs, scale=c, loss_identity=loss_identity,
+               epochs=epochs, scale=c, norm=norm, loss_identity=loss_identity,
                loss_first=float(np.mean(hist[:20])), loss_last=float(np.mean(hist[-20:])))
     log(f"[cfg5] stage {stage}: pairs {t_pairs:.0f}s, training {t_train:.0f}s, loss "
         f"{res['loss_first']:.4e} -> {res['loss_last']:.4e} (scaled image {loss_identity:.4e})")

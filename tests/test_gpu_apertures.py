@@ -152,3 +152,70 @@ def test_new_geometry_drops_model_windows():
         ctx.set_shot_model(i, s["case"]["model"]["v"], s["case"]["model"]["r1"])
     ctx.forward_begin()
     ctx.close()
+
+
+def test_shared_model_windows():
+    """Shots whose model windows are equal share one stored window: a batch
+    of six shots over two windows (shots 0, 2, 4 on window A and 1, 3, 5 on
+    window B, different sources) equals its shots' one-shot runs through a
+    Born / adjoint pair with LS rows (the transposed stencil
+    reads the shared window's tap planes); re-pointing shot 5 at window A
+    then gives shot 5 the results of a one-shot run on A."""
+    from rtm_gpu_amd import hip
+    shots, _ = _problem()
+    A, B = shots[0], shots[2]  # widths 60 and 80
+    order = [A, B, A, B, A, B]
+    depths = [10, 12, 15, 17, 20, 22]  # source rows
+    wmax = max(s["wd"] for s in order)
+    base = {k: A["case"]["cfg"][k] for k in _keys()}
+    nxmax = wmax + 2 * N2
+    nt = base["nt"]
+
+    def batch_ctx():
+        ctx = hip.Context(**base, nx=nxmax, s_l=N2, s_r=N2 + wmax - 1, n=wmax, max_shots=6,
+                          stack_nx=MOD_NX, record=1)
+        m = A["case"]["model"]
+        vflat = np.full((base["nz"], nxmax), m["vmin"], np.float32)
+        ctx.set_model(vflat, vflat * 0, m["w"], m["c"], m["index"], m["nvel"], m["vmin"], m["dv"])
+        ctx.set_wavelet(m["wav"])
+        ctx.set_shot_geometry([dict(nx=s["case"]["cfg"]["nx"], r_u=depths[i],
+                                    r_x=s["case"]["r_x"][0], s_l=s["case"]["cfg"]["s_l"],
+                                    s_r=s["case"]["cfg"]["s_r"], n=s["case"]["cfg"]["n"],
+                                    x_img=s["xb"]) for i, s in enumerate(order)])
+        for i, s in enumerate(order):
+            ctx.set_shot_model(i, s["case"]["model"]["v"], s["case"]["model"]["r1"])
+        return ctx
+
+    def one(s, depth):
+        case = dict(s["case"])
+        case["r_u"] = [depth]
+        return cases.hip_context(case, nshot=1, record=1)
+
+    ctx = batch_ctx()
+    rng = np.random.default_rng(5)
+    seis = np.zeros((6, wmax, nt), np.float32)
+    for i, s in enumerate(order):
+        seis[i, :s["wd"]] = rng.standard_normal((s["wd"], nt)).astype(np.float32) * 1e-3
+    refl = (rng.standard_normal((MOD_NZ, MOD_NX)) * 0.02).astype(np.float32)
+    ctx.set_reflectivity(refl)
+    born = ctx.born()
+    grad = ctx.adjoint(seis)
+    exp = np.zeros((MOD_NZ, MOD_NX), np.float32)
+    for i, s in enumerate(order):
+        wd, xb = s["wd"], s["xb"]
+        o = one(s, depths[i])
+        o.set_reflectivity(refl[:, xb:xb + wd])
+        assert np.array_equal(born[i, :wd], o.born()[0]), f"shot {i} Born data"
+        exp[:, xb:xb + wd] = exp[:, xb:xb + wd] + o.adjoint(seis[i:i + 1, :wd])
+        o.close()
+    assert np.array_equal(grad, exp)
+    # shot 5 leaves window B for a window of its own (B 1% faster), then
+    # returns: the other shots keep their results throughout
+    mB = order[5]["case"]["model"]
+    ctx.set_shot_model(5, mB["v"] * np.float32(1.01), mB["r1"])
+    born2 = ctx.born()
+    assert np.array_equal(born2[:5], born[:5])
+    assert not np.array_equal(born2[5], born[5])
+    ctx.set_shot_model(5, mB["v"], mB["r1"])
+    assert np.array_equal(ctx.born(), born)
+    ctx.close()

@@ -124,7 +124,8 @@ def test_cfg5_pipeline_full_size():
     assert c > 0
     net = cnn.Deblur(list(deblur.CFG5_CHANNELS), 256 * 1024, 256)
     net.init_params(seed=0)  # zero last layer: output c * image
-    res = deblur.cfg5_evaluate(S, net, c, neval=2, panels=2, cg_iters=2)
+    norm = float(np.sqrt(np.mean(M.astype(np.float64) ** 2)))
+    res = deblur.cfg5_evaluate(S, net, c, norm, neval=2, panels=2, cg_iters=2)
     print({k: v for k, v in res.items() if k != "per_model"})
     for q in res["per_model"]:
         assert q["choice"] == "rtm"
