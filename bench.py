@@ -552,6 +552,50 @@ def run_single(a):
     print(json.dumps(out), flush=True)
 
 
+def run_cnn(a):
+    """The deblurring network of the random-reflectivity configuration alone:
+    six 3x3 layers of 32 channels, BF16 MFMA with FP32 accumulation, trained
+    on random 64 x 64 patches in batches of 256 (Adam, lr 1e-3), and applied
+    to a whole 256 x 1024 image.  Synthetic inputs (the kernels' speed does
+    not depend on the values)."""
+    from rtm_gpu_amd import cnn
+    ch = [3] + [32] * 5 + [1]
+    B, P, N = 256, 64, 2048
+    net = cnn.Deblur(ch, B * P * P, B * P)
+    net.init_params(0)
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((N, 3, P, P)).astype(np.float32)
+    t = rng.standard_normal((N, P, P)).astype(np.float32)
+    for _ in range(max(a.warmup, 1)):
+        net.fit(x, t, 1, B, 1e-3)
+    epochs = max(a.steps, 1)
+    t0 = time.perf_counter()
+    net.fit(x, t, epochs, B, 1e-3)
+    el = time.perf_counter() - t0
+    steps = epochs * (N // B)
+    mac_f = sum(ch[i] * ch[i + 1] * 9 for i in range(len(ch) - 1))
+    mac_b = mac_f + sum(ch[i] * ch[i + 1] * 9 for i in range(1, len(ch) - 1))
+    flop = 2.0 * B * P * P * (mac_f + mac_b)  # forward, weight and input gradients
+    xi = rng.standard_normal((1, 3, 256, 1024)).astype(np.float32)
+    net.forward(xi)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        net.forward(xi)
+    t_inf = (time.perf_counter() - t0) / 5
+    tf = flop / (el / steps) / 1e12
+    out = {"metric": "deblurring network training step: TFLOP/s (BF16 MFMA)",
+           "value": round(tf, 2), "unit": "TFLOP/s", "n_gpus": 1, "steps": steps,
+           "warmup": max(a.warmup, 1), "ms_per_step": round(el / steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16 (fp32 accumulation)", "data": "synthetic",
+           "config": {"workload": "cnn_6x32ch_3x3_patch64_batch256",
+                      "channels": ch, "patch": P, "batch": B},
+           "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": 2500.0,
+                        "unit": "TFLOP/s", "frac": round(tf / 2500.0, 4), "traffic": None},
+           "inference_256x1024_ms": round(t_inf * 1e3, 3)}
+    print(json.dumps(out), flush=True)
+
+
 def run_deblur(a):
     """The random-reflectivity deblurring configuration (1024 x 256 at 10 m,
     64 shots, 25 Hz, 3000 steps, per-model backgrounds): the trained network
@@ -673,13 +717,17 @@ def main():
     ap.add_argument("--shots", type=int, default=128)
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--strips", default="f32", choices=["f32", "f16", "bf16"])
-    ap.add_argument("--zseg", type=int, default=0)
+    ap.add_argument("--zseg", type=int, default=0,
+                    help="rows per wave of every interior kernel (0: autotune)")
+    ap.add_argument("--zsegs", default="",
+                    help="rows per wave of the forward, reconstruction and receiver-step "
+                         "kernels, comma-separated (instead of the autotuner; for profiles)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--host-gathers", action="store_true",
                     help="every timed step uploads the gathers from host memory (the PCIe-"
                          "inclusive rate of a caller that hands over host buffers)")
     ap.add_argument("--workload", default="rvsp",
-                    choices=["rvsp", "single", "apertures", "deblur", "lsrtm"],
+                    choices=["rvsp", "single", "apertures", "deblur", "lsrtm", "cnn"],
                     help="rvsp: the headline reverse-VSP run; single: the single-shot "
                          "200 x 400 layered run, eager and as a replayed HIP graph; "
                          "apertures: uneven-aperture "
@@ -687,7 +735,8 @@ def main():
                          "trained deblurring network against 10-iteration CGLS least-squares "
                          "migration on held-out random-reflectivity models (about 6 minutes "
                          "per 8 models); lsrtm: "
-                         "CGLS iterations on the reverse-VSP batch (--steps iterations)")
+                         "CGLS iterations on the reverse-VSP batch (--steps iterations); cnn: "
+                         "the deblurring network's training step and inference alone")
     ap.add_argument("--batch", type=int, default=128, help="apertures: shots per launch")
     ap.add_argument("--neval", type=int, default=8,
                     help="deblur: held-out models evaluated (8 per launch; 32 = the full set)")
@@ -707,6 +756,9 @@ def main():
         return
     if a.workload == "single":
         run_single(a)
+        return
+    if a.workload == "cnn":
+        run_cnn(a)
         return
     if a.workload == "lsrtm":
         run_lsrtm(a)
@@ -740,11 +792,17 @@ def main():
     ctx.sync()
     ctx.set_model(mm["v"], mm["r1"], P["w"], mm["c"], mm["index"], mm["nvel"], mm["vmin"], 1.0)
     zseg = a.zseg
-    if zseg <= 0:
+    if a.zsegs:  # forward, reconstruction, receiver step[, frame top/bottom, frame sides]
+        zs = [int(v) for v in a.zsegs.split(",")]
+        zseg = zs[:3]
+        ctx.set_zseg(*zseg)
+        if len(zs) == 5:
+            ctx.set_frame_runs(zs[3], zs[4])
+    elif zseg <= 0:
         t = time.time()
         zseg = ctx.autotune(8)
-        log(f"[bench] autotune (forward, reconstruction, receiver step): {zseg} rows per wave "
-            f"({time.time() - t:.1f}s)")
+        log(f"[bench] autotune (forward, reconstruction, receiver step): {zseg} rows per wave; "
+            f"frame runs (top/bottom, sides) {ctx.frame_runs()} ({time.time() - t:.1f}s)")
     log(f"[bench] setup + modelling {time.time() - t_setup:.1f}s")
 
     host_seis = ctx.records() if a.host_gathers else None  # [shot][n][nt], trace-major
@@ -812,7 +870,8 @@ def main():
         "config": {"workload": workload, "grid": [d.nz, d.nx], "interior": [d.nz - 60, d.nx - 60],
                    "nt": d.nt, "shots": a.shots, "nfdmax": cfg.nfdmax, "n2": cfg.n2,
                    "strips": a.strips, "coefficients": "adaptive LS",
-                   "velocity_bins": mm["nbins"], "rows_per_wave": zseg},
+                   "velocity_bins": mm["nbins"], "rows_per_wave": zseg,
+                   "frame_runs": list(ctx.frame_runs())},
         "shots_per_s": round(a.shots * a.steps / el, 3),
         "hbm_frac_whole_job": round(total_bytes / el / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(kd["gbs"], 1),

@@ -255,20 +255,29 @@ int rtm_data_dot_noalloc(const float* x, const float* y, long long n, long long 
 
 /* Autotuner (not in the reference): for each interior kernel of the
  * migration (forward, source reconstruction, receiver step + imaging) times
- * that kernel inside nsteps forward + backward step pairs of the current batch
- * (the other kernels interleaved, the buffers rotating as in a migration) for
- * each candidate height of its row segments (rounded by the kernel to a
- * multiple of its register-queue length), one kernel at a time, and keeps the
- * fastest per kernel for this context (Born modelling and the adjoint take the
- * receiver step's); zseg[3], when not NULL, receives the three choices.  Needs
+ * that kernel inside forward + backward step pairs of the current batch (the
+ * other kernels interleaved, the buffers rotating as in a migration) for row
+ * segments of 1, 2, 3, 4, 6 and 9 times its register-queue length, in three
+ * interleaved rounds of max(2, nsteps/2) timed pairs per candidate, one
+ * kernel at a time, and keeps the fastest per kernel for this context (Born
+ * modelling and the adjoint take the receiver step's); zseg[3], when not
+ * NULL, receives the three choices in rows.  Needs
  * model, wavelet and shots; leaves the passes to be begun anew and resets the
  * rtm_kernel_time counters.  Rows per wave trade the vertical halo re-read
- * (2*NFD rows per segment) against the number of waves in flight. */
+ * (2*NFD rows per segment) against the number of waves in flight.  Then the
+ * frame kernels' row runs (rtm_set_frame_runs) are timed the same way, top and
+ * bottom bands first, on the sum of both frame kernels' times. */
 int rtm_autotune(rtm_ctx* ctx, int nsteps, int* zseg);
 /* The autotuner's three choices set by hand: rows per wave of the forward,
  * the source-reconstruction and the receiver-step kernels (Born and the
  * adjoint take the receiver step's). */
 int rtm_set_zseg(rtm_ctx* ctx, const int* zseg);
+/* Rows per frame-kernel wave: the top and bottom bands run in pieces of
+ * rows_tb rows (ceil(n2/3)..n2), the side bands in pieces of rows_side rows
+ * (from half the default up to nz).  rtm_autotune picks both; rtm_frame_runs
+ * reads the current pair. */
+int rtm_set_frame_runs(rtm_ctx* ctx, int rows_tb, int rows_side);
+int rtm_frame_runs(rtm_ctx* ctx, int* rows_tb, int* rows_side);
 
 /* Kernel timing with HIP events on the context stream: mask is a bit set of
  * kernel kinds (1 << RTM_K_*), 0 = off, -1 = all.  Resets the counters. */

@@ -56,6 +56,7 @@ struct rtm_ctx {
   int gw = 0;                  // columns of the stacked image
   int4* d_tiles = nullptr;     // frame (tile, shot) records of the batch
   int ntiles = 0, tile_cap = 0;
+  int fseg_tb = 0, fseg_side = 0;  // rows per frame record: top/bottom bands, side bands
   // model windows of a ragged batch (allocated on first use): slot w holds
   // one distinct window; shots whose windows are equal share a slot
   float *d_mv = nullptr, *d_mr1 = nullptr, *d_mpm = nullptr;
@@ -461,27 +462,28 @@ int adjoint_step(rtm_ctx* c, int k) {
   return adjoint_update(c, k, true);
 }
 
-// Frame records (k_frame) of a grid nx columns wide, tagged with the shot.
-void build_tiles(const Grid& g, int nx, int shot, std::vector<int4>& t) {
-  const int seg = frame_segment_rows(g.nfd);
+// Frame records (k_frame) of a grid nx columns wide, tagged with the shot:
+// top and bottom bands in runs of seg_tb rows, side bands in runs of seg.
+void build_tiles(const Grid& g, int nx, int shot, int seg_tb, int seg, std::vector<int4>& t) {
   int n = 0;
-  frame_records(g, nx, shot, seg, nullptr, &n);
+  frame_records(g, nx, shot, seg_tb, seg, nullptr, &n);
   const size_t at = t.size();
   t.resize(at + n);
-  frame_records(g, nx, shot, seg, t.data() + at, &n);
+  frame_records(g, nx, shot, seg_tb, seg, t.data() + at, &n);
 }
 
 int choose_zs(rtm_ctx* c, int nshot);
 void drop_graph(rtm_ctx* c);
 
-// Upload the batch geometry and its frame records: tile-major with the shot
-// fastest, so that one XCD holds many shots of the same tiles at a time.
-int upload_geometry(rtm_ctx* c) {
+// The batch's frame records for the context's run lengths: tile-major with
+// the shot fastest, so that one XCD holds many shots of the same tiles at a
+// time.
+int upload_frame_records(rtm_ctx* c) {
   const int ns = (int)c->geo.size();
   std::vector<std::vector<int4>> per(ns);
   size_t most = 0;
   for (int s = 0; s < ns; ++s) {
-    build_tiles(c->g, c->geo[s].nx, s, per[s]);
+    build_tiles(c->g, c->geo[s].nx, s, c->fseg_tb, c->fseg_side, per[s]);
     most = std::max(most, per[s].size());
   }
   std::vector<int4> rec;
@@ -489,10 +491,19 @@ int upload_geometry(rtm_ctx* c) {
     for (int s = 0; s < ns; ++s)
       if (t < per[s].size()) rec.push_back(per[s][t]);
   if ((int)rec.size() > c->tile_cap) return fail(c, RTM_E_ARG, "frame records exceed capacity");
-  HIPCHK(c, hipMemcpyAsync(c->d_geo, c->geo.data(), ns * sizeof(ShotGeo), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_tiles, rec.data(), rec.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->ntiles = (int)rec.size();
+  drop_graph(c);
+  return RTM_OK;
+}
+
+// Upload the batch geometry and its frame records.
+int upload_geometry(rtm_ctx* c) {
+  const int ns = (int)c->geo.size();
+  HIPCHK(c, hipMemcpyAsync(c->d_geo, c->geo.data(), ns * sizeof(ShotGeo), hipMemcpyHostToDevice, c->stream));
+  int rc = upload_frame_records(c);
+  if (rc != RTM_OK) return rc;
   c->nshot = ns;
   choose_zs(c, ns);
   drop_graph(c);
@@ -672,8 +683,11 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
     rc = fail(c, RTM_E_NOMEM, "hipHostMalloc of the watch flag failed");
   }
   {
-    std::vector<int4> tiles;  // the widest grid has the most frame tiles
-    build_tiles(g, g.nx, 0, tiles);
+    // the widest grid with the shortest runs the autotuner tries has the most records
+    c->fseg_tb = g.n2;
+    c->fseg_side = frame_segment_rows(g.nfd);
+    std::vector<int4> tiles;
+    build_tiles(g, g.nx, 0, std::max(1, (g.n2 + 2) / 3), std::max(1, c->fseg_side / 2), tiles);
     c->tile_cap = (int)tiles.size() * p->max_shots;
   }
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_tiles, (size_t)c->tile_cap);
@@ -1364,11 +1378,33 @@ int rtm_set_zseg(rtm_ctx* c, const int* zseg) {
   return RTM_OK;
 }
 
+int rtm_set_frame_runs(rtm_ctx* c, int rows_tb, int rows_side) {
+  if (!c) return RTM_E_ARG;
+  const Grid& g = c->g;
+  const int min_tb = std::max(1, (g.n2 + 2) / 3), min_side = std::max(1, frame_segment_rows(g.nfd) / 2);
+  if (rows_tb < min_tb || rows_tb > g.n2 || rows_side < min_side || rows_side > g.nz)
+    return fail(c, RTM_E_ARG, "frame runs %d / %d outside %d..%d / %d..%d", rows_tb, rows_side,
+                min_tb, g.n2, min_side, g.nz);
+  c->fseg_tb = rows_tb;
+  c->fseg_side = rows_side;
+  return c->geo.empty() ? RTM_OK : upload_frame_records(c);
+}
+
+int rtm_frame_runs(rtm_ctx* c, int* rows_tb, int* rows_side) {
+  if (!c || !rows_tb || !rows_side) return RTM_E_ARG;
+  *rows_tb = c->fseg_tb;
+  *rows_side = c->fseg_side;
+  return RTM_OK;
+}
+
 int rtm_autotune(rtm_ctx* c, int nsteps, int* zseg) {
   if (!c || nsteps < 1) return RTM_E_ARG;
   if (!c->model_ok || !c->wav_ok || !c->shots_ok)
     return fail(c, RTM_E_STATE, "model, wavelet and shots must be set");
-  static const int kCand[] = {8, 16, 24, 32, 48, 64, 96};
+  // candidate heights: 1, 2, 3, 4, 6 and 9 register-queue lengths of the
+  // kernel (each kernel rounds a request to a multiple of its queue)
+  static const int kMult[] = {1, 2, 3, 4, 6, 9};
+  constexpr int kNC = 6, kRounds = 3;
   // Each interior kernel kind of the migration is timed where it runs: inside
   // forward and backward step pairs on the batch's own buffers (the frame,
   // strip and imaging kernels interleaved, the buffers rotating), one kind at a
@@ -1402,25 +1438,71 @@ int rtm_autotune(rtm_ctx* c, int nsteps, int* zseg) {
     }
     return RTM_OK;
   };
+  // One measurement: an untimed pair, then per-launch time over `per` pairs.
+  // The candidates are measured in kRounds interleaved rounds, alternating in
+  // direction, and compared on their summed times, so that drift of the
+  // device (clocks, caches, the other kernels) falls on all of them alike.
+  const int per = std::max(2, nsteps / 2);
+  auto measure = [&](unsigned mask, double* ms) -> int {
+    c->timing = 0;
+    int r = pairs(1);
+    if (r != RTM_OK) return r;
+    c->timing = (int)mask;
+    for (int k = 0; k < RTM_K_NKINDS; ++k)
+      if (mask & (1u << k)) { c->t_ms[k] = 0.0; c->t_n[k] = 0; }
+    r = pairs(per);
+    if (r == RTM_OK) r = drain_timing(c);
+    if (r != RTM_OK) return r;
+    *ms = 0.0;
+    for (int k = 0; k < RTM_K_NKINDS; ++k)
+      if (mask & (1u << k)) *ms += c->t_n[k] ? c->t_ms[k] / c->t_n[k] : 1e300;
+    return RTM_OK;
+  };
   for (int ki = 0; ki < 3 && rc == RTM_OK; ++ki) {
-    const int epi = kinds[ki], kind = timers[ki];
-    int best = c->zs_epi[epi];
-    double best_ms = 1e300;
-    for (int z : kCand) {
-      c->zs_epi[epi] = z;
-      c->timing = 0;
-      rc = pairs(1);  // untimed
-      if (rc != RTM_OK) break;
-      c->timing = 1 << kind;
-      c->t_ms[kind] = 0.0;
-      c->t_n[kind] = 0;
-      rc = pairs(nsteps);
-      if (rc == RTM_OK) rc = drain_timing(c);
-      if (rc != RTM_OK) break;
-      const double ms = c->t_n[kind] ? c->t_ms[kind] / c->t_n[kind] : 1e300;
-      if (ms < best_ms) { best_ms = ms; best = z; }
+    const int epi = kinds[ki];
+    const int ql = interior_queue_rows(epi, c->nfd);
+    double tot[kNC] = {0.0};
+    for (int round = 0; round < kRounds && rc == RTM_OK; ++round)
+      for (int i = 0; i < kNC && rc == RTM_OK; ++i) {
+        const int ci = (round & 1) ? kNC - 1 - i : i;
+        c->zs_epi[epi] = kMult[ci] * ql;
+        double ms = 0.0;
+        rc = measure(1u << timers[ki], &ms);
+        tot[ci] += ms;
+      }
+    int bi = 0;
+    for (int i = 1; i < kNC; ++i)
+      if (tot[i] < tot[bi]) bi = i;
+    c->zs_epi[epi] = kMult[bi] * ql;
+  }
+  // Frame record runs, timed on both frame kernels of the step pair: the top
+  // and bottom bands in runs of n2, n2/2 or n2/3 rows, then the side bands in
+  // runs of half, once or twice the default.  Shorter runs put more waves in
+  // flight at the cost of recomputing two pre-blend rows per run.
+  if (rc == RTM_OK) {
+    const Grid& g = c->g;
+    const int seg0 = frame_segment_rows(g.nfd);
+    const int cand[2][3] = {{g.n2, (g.n2 + 1) / 2, (g.n2 + 2) / 3},
+                            {seg0, std::max(1, seg0 / 2), 2 * seg0}};
+    const unsigned fmask = (1u << RTM_K_FWD_FRAME) | (1u << RTM_K_RCV_FRAME);
+    for (int which = 0; which < 2 && rc == RTM_OK; ++which) {
+      int& run = which == 0 ? c->fseg_tb : c->fseg_side;
+      double tot[3] = {0.0, 0.0, 0.0};
+      for (int round = 0; round < kRounds && rc == RTM_OK; ++round)
+        for (int i = 0; i < 3 && rc == RTM_OK; ++i) {
+          const int ci = (round & 1) ? 2 - i : i;
+          run = cand[which][ci];
+          rc = upload_frame_records(c);
+          double ms = 0.0;
+          if (rc == RTM_OK) rc = measure(fmask, &ms);
+          tot[ci] += ms;
+        }
+      int bi = 0;
+      for (int i = 1; i < 3; ++i)
+        if (tot[i] < tot[bi]) bi = i;
+      run = cand[which][bi];
     }
-    c->zs_epi[epi] = best;
+    if (rc == RTM_OK) rc = upload_frame_records(c);
   }
   c->timing = saved_timing;
   c->check_every = saved_check;

@@ -139,7 +139,9 @@ hipError_t launch_frame(int mode, int nfd_inst, FrameArgs a, int nshot, int nrec
 // Frame records of one shot of grid width nx (k_frame): the top and bottom
 // bands in runs of 64 columns (62 written), the left and right bands in runs
 // of 64 columns by `seg` rows.
-void frame_records(const Grid& g, int nx, int shot, int seg, int4* out, int* count);
+// Frame records of one shot: top and bottom bands in runs of seg_tb rows,
+// the side bands in runs of seg rows; 62 written columns per record.
+void frame_records(const Grid& g, int nx, int shot, int seg_tb, int seg, int4* out, int* count);
 int frame_segment_rows(int nfd);
 hipError_t launch_fill(float* p, long long n, float val, hipStream_t s);
 // Neighbour-tap planes (Model::tp) of one model window from its coefficient

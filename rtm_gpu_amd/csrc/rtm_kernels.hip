@@ -594,6 +594,14 @@ __device__ __forceinline__ unsigned bld1u(Rsrc r, unsigned vo, unsigned so) {
 __device__ __forceinline__ void bst1(Rsrc r, unsigned vo, unsigned so, float v) {
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(vo + so), 0, 0);
 }
+// streaming variants (u0 rows, u(k+1) and strip writes: touched once per step)
+__device__ __forceinline__ float bld1s(Rsrc r, unsigned vo, unsigned so) {
+  return __uint_as_float((unsigned)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so,
+                                                                        RTM_STREAM_POLICY));
+}
+__device__ __forceinline__ void bst1s(Rsrc r, unsigned vo, unsigned so, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(vo + so), 0, RTM_STREAM_POLICY);
+}
 
 // Strip element of frame cell (z, x) of a shot nx wide, or -1 (strip_index
 // without branches, in 32 bits: a slot holds fewer than 2^31 elements).
@@ -746,10 +754,10 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   for (int d = 0; d < PD; ++d) {
     pmb[d] = bld1(RP, vcol, rowm(t0 + d));
     halb[d] = bld1(R1, vhal, rowm(t0 + d));
-    u0b[2 + d] = bld1(R0, vcol, rowm(t0 + d));   // slot (it + 2) % 6 holds row t0 + it
+    u0b[2 + d] = bld1s(R0, vcol, rowm(t0 + d));   // slot (it + 2) % 6 holds row t0 + it
   }
-  u0b[0] = bld1(R0, vcol, rowm(t0 - 2));
-  u0b[1] = bld1(R0, vcol, rowm(t0 - 1));
+  u0b[0] = bld1s(R0, vcol, rowm(t0 - 2));
+  u0b[1] = bld1s(R0, vcol, rowm(t0 - 1));
   CoefRow<NFD> cr[2];
   cr[0].load(RC, kob[0]);
   float ext[2][NE];
@@ -771,7 +779,7 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
       q[(j + NB + NFD + PD) % U] = bld1(R1, vcol, rowm(tt + NFD + PD));
       pmb[(j + PD) % 3] = bld1(RP, vcol, rowm(tt + PD));
       halb[(j + PD) % 3] = bld1(R1, vhal, rowm(tt + PD));
-      u0b[(j + 2 + PD) % 6] = bld1(R0, vcol, rowm(tt + PD));
+      u0b[(j + 2 + PD) % 6] = bld1s(R0, vcol, rowm(tt + PD));
       abc_ops(tt + 1, vpb[(j + 2) % 3], r1b[(j + 2) % 3], vqb[(j + 2) % 3], wb[(j + 2) % 3]);
       t_ops(tt + 1, ext[(j + 1) % 2]);
       __builtin_amdgcn_sched_barrier(0);
@@ -855,10 +863,10 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
         const float O = (1.0f / (tv + 1.0f)) * ((t1 - t2) + t3);
         const float val = (1.0f - wgt) * T0 + wgt * O;
         const bool out = outcol && it >= 2 && z < z0 + nrows;
-        bst1(R2, out ? vcol : kDrop, rowm(z), val);
+        bst1s(R2, out ? vcol : kDrop, rowm(z), val);
         if (MODE == FR_FWD) {  // Hybrid3 (kernel.cu:184-208)
           const int e = strip_elem(g, NX, sg.mnx, z, x);
-          bst1(RS, (out && e >= 0) ? (unsigned)e * 4u : kDrop, 0u, val);
+          bst1s(RS, (out && e >= 0) ? (unsigned)e * 4u : kDrop, 0u, val);
         }
       } else {
         const int z = tt - 1;
@@ -901,10 +909,10 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
         const float O = (1.0f / (corner ? den_c : den_s)) * (corner ? num_c : num_s);
         const float val = (1.0f - wgt) * T0 + wgt * O;
         const bool out = outcol && it >= 2 && z < z0 + nrows;
-        bst1(R2, out ? vcol : kDrop, rowm(z), val);
+        bst1s(R2, out ? vcol : kDrop, rowm(z), val);
         if (MODE == FR_FWD) {  // Hybrid3 (kernel.cu:184-208)
           const int e = strip_elem(g, NX, sg.mnx, z, x);
-          bst1(RS, (out && e >= 0) ? (unsigned)e * 4u : kDrop, 0u, val);
+          bst1s(RS, (out && e >= 0) ? (unsigned)e * 4u : kDrop, 0u, val);
         }
       }
     }
@@ -1352,14 +1360,16 @@ int frame_segment_rows(int nfd) {
   const int nb = nfd > 2 ? nfd : 2;
   return 2 * (nb + nfd + 1 + 2) - 2;  // two register-queue cycles per side run
 }
-void frame_records(const Grid& g, int nx, int shot, int seg, int4* out, int* count) {
+void frame_records(const Grid& g, int nx, int shot, int seg_tb, int seg, int4* out, int* count) {
   int n = 0;
   auto put = [&](int z0, int cx0, int band, int rows) {
     if (out) out[n] = make_int4(z0, cx0, band | (rows << 4), shot);
     ++n;
   };
-  for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(0, cx0, 0, g.n2);
-  for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(g.nz - g.n2, cx0, 1, g.n2);
+  for (int z0 = 0; z0 < g.n2; z0 += seg_tb)  // top band in runs of seg_tb rows
+    for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, 0, std::min(seg_tb, g.n2 - z0));
+  for (int z0 = g.nz - g.n2; z0 < g.nz; z0 += seg_tb)
+    for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, 1, std::min(seg_tb, g.nz - z0));
   for (int z0 = g.n2; z0 < g.nz - g.n2; z0 += seg) {
     const int rows = std::min(seg, g.nz - g.n2 - z0);
     for (int cx0 = -1; cx0 + 1 < g.n2; cx0 += 62) put(z0, cx0, 2, rows);

@@ -98,6 +98,10 @@ def lib():
         L.rtm_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         L.rtm_set_zseg.argtypes = [vp, C.POINTER(C.c_int)]
         L.rtm_set_zseg.restype = C.c_int
+        L.rtm_set_frame_runs.argtypes = [vp, C.c_int, C.c_int]
+        L.rtm_set_frame_runs.restype = C.c_int
+        L.rtm_frame_runs.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.rtm_frame_runs.restype = C.c_int
         L.rtm_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         for n in ("rtm_create", "rtm_destroy", "rtm_set_model", "rtm_set_wavelet", "rtm_set_shots",
                   "rtm_set_shot_geometry", "rtm_set_shot_model",
@@ -313,6 +317,16 @@ class Context:
         """Rows per wave of the forward, reconstruction and receiver-step kernels."""
         z = (C.c_int * 3)(int(fwd), int(src), int(rcv))
         self._chk(lib().rtm_set_zseg(self.h, z), "rtm_set_zseg")
+
+    def set_frame_runs(self, rows_tb, rows_side):
+        """Rows per frame-kernel wave: top/bottom bands, side bands."""
+        self._chk(lib().rtm_set_frame_runs(self.h, int(rows_tb), int(rows_side)),
+                  "rtm_set_frame_runs")
+
+    def frame_runs(self):
+        a, b = C.c_int(), C.c_int()
+        self._chk(lib().rtm_frame_runs(self.h, C.byref(a), C.byref(b)), "rtm_frame_runs")
+        return a.value, b.value
 
     def set_timing(self, mask):
         """mask: bit set of kernel kinds (1 << K_*), 0 off, -1 all."""
