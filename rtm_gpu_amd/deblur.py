@@ -515,7 +515,7 @@ def cfg5_evaluate(S, net, c, norm, neval=32, panels=8, cg_iters=10, seed0=CFG5_E
         ctx.sync()
         times["guard"] += time.time() - t
         t = time.time()
-        iters = _Iterates()
+        iters = _Iterates(f"[cfg5] held-out {b}..{b + k - 1}: CGLS iteration")
         m_cg, hist = lsrtm.cgls(op, None, cg_iters, callback=iters)
         times["cg"] += time.time() - t
         reach = iters.first_within(m_cg, axis_models=True)
