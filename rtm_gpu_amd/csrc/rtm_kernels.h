@@ -113,7 +113,8 @@ struct FrameArgs {
   const float* u0;
   float* u2;
   // one record per wave: (first output row, column of lane 0, band | rows << 4,
-  // shot); band 0 top, 1 bottom, 2 left, 3 right (frame_records)
+  // shot); band 0 top, 1 bottom, 2 left, 3 right, 4 both sides in one wave
+  // (frame_records)
   const int4* recs;
   const ShotGeo* geo;
   float wav;

@@ -624,7 +624,9 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   // t-NB .. t+NFD+PD; every other per-row buffer is circular with a length
   // dividing 12 (2, 3 or 6), so no buffer is ever copied, only renamed.
   constexpr int U = (NB + NFD + 1 + PD + 11) / 12 * 12;
-  constexpr int WN = 64 + 2 * NFD + 1;    // LDS row: halo | 64 columns | halo | dump slot
+  // LDS row: halo | 64 columns | halo | dump slot; a packed side record
+  // (band 4) uses halo | 32 columns | halo | halo | 32 columns | halo | dump
+  constexpr int WN = 64 + 4 * NFD + 1;
   __shared__ float win[4][WN];
   const Grid& g = a.g;
   const Model& m = a.m;
@@ -636,12 +638,23 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   const int z0 = rec.x, cx0 = rec.y, band = rec.z & 15, nrows = rec.z >> 4, shot = rec.w;
   const ShotGeo sg = a.geo[shot];
   const int NX = sg.nx, NZ = g.nz, N2 = g.n2;
-  const int x = cx0 + lane;
+  // Band 4 packs both side bands of a row run into one wave (n2 <= 30):
+  // lanes 0..31 hold columns -1..30 (the left band and its inner
+  // neighbour), lanes 32..63 columns nx-31..nx; lanes 1..30 and 33..62
+  // write.  The +-1 lane shuffles of the one-way update stay inside a half.
+  const bool packed = band == 4;
+  const bool rhalf = packed && lane >= 32;
+  const int x = packed ? (rhalf ? NX - 63 + lane : lane - 1) : cx0 + lane;
   const int xm = mirror_idx(x, NX);
   int ccl = 0, ccr = NX;  // output columns of the region
   if (band == 2) ccr = N2;
   else if (band == 3) ccl = NX - N2;
-  const bool outcol = lane >= 1 && lane <= 62 && x >= ccl && x < ccr;
+  else if (packed) { ccl = rhalf ? NX - N2 : 0; ccr = rhalf ? NX : N2; }
+  const int hl_ = packed ? (lane & 31) : lane;  // lane within its (half) run
+  const bool outcol = hl_ >= 1 && hl_ <= (packed ? 30 : 62) && x >= ccl && x < ccr;
+  // LDS slot of the lane's column (packed: the right half after the left
+  // half's right halo and its own left halo)
+  const int slot = packed && rhalf ? 3 * NFD + lane : NFD + lane;
   // A top or bottom run whose written columns all lie in [n2+1, nx-n2-2]
   // holds only cells of the top (bottom) edges: no corner cell, no side cell
   // (the ring of (z, x) is z, resp. nz-1-z).  Those waves take a shorter
@@ -662,11 +675,20 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   const unsigned vcol = colb(xm);
 
   // halo columns of the LDS row: lanes 0..NFD-1 the left ones, NFD..2NFD-1
-  // the right ones; the other lanes fetch nothing and write the dump slot
+  // the right ones (packed: the left half's two halos, then lanes
+  // 2NFD..4NFD-1 the right half's); the other lanes fetch nothing and write
+  // the dump slot
   unsigned vhal = kDrop;
   int hpos = WN - 1;
-  if (lane < NFD) { vhal = colb(mirror_idx(cx0 - NFD + lane, NX)); hpos = lane; }
-  else if (lane < 2 * NFD) { vhal = colb(mirror_idx(cx0 + 64 + lane - NFD, NX)); hpos = 64 + lane; }
+  if (!packed) {
+    if (lane < NFD) { vhal = colb(mirror_idx(cx0 - NFD + lane, NX)); hpos = lane; }
+    else if (lane < 2 * NFD) { vhal = colb(mirror_idx(cx0 + 64 + lane - NFD, NX)); hpos = 64 + lane; }
+  } else {
+    if (lane < NFD) { vhal = colb(mirror_idx(-1 - NFD + lane, NX)); hpos = lane; }
+    else if (lane < 2 * NFD) { vhal = colb(mirror_idx(31 + lane - NFD, NX)); hpos = 32 + lane; }
+    else if (lane < 3 * NFD) { vhal = colb(mirror_idx(NX - 31 - 3 * NFD + lane, NX)); hpos = 32 + lane; }
+    else if (lane < 4 * NFD) { vhal = colb(mirror_idx(NX + 1 - 3 * NFD + lane, NX)); hpos = 64 + lane; }
+  }
 
   // receivers on row s_z (kernel.cu:349): this step's data sample of the lane's
   // trace (receiver field: sample k+1; adjoint: sample k), dropped elsewhere
@@ -785,14 +807,14 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
       __builtin_amdgcn_sched_barrier(0);
       // (3) horizontal neighbours of row tt through LDS
       const float u = q[(j + NB) % U];
-      win[wv][NFD + lane] = u;
+      win[wv][slot] = u;
       win[wv][hpos] = halb[j % 3];
       wave_lds_sync();
       float hl[NFD], hr[NFD];
 #pragma unroll
       for (int l = 1; l <= NFD; ++l) {
-        hl[l - 1] = win[wv][NFD + lane - l];
-        hr[l - 1] = win[wv][NFD + lane + l];
+        hl[l - 1] = win[wv][slot - l];
+        hr[l - 1] = win[wv][slot + l];
       }
       wave_lds_sync();
       // (4) pre-blend two-way value T of (tt, x) (Add / BKAdd, kernel.cu:46-80, 339-380)
@@ -1370,8 +1392,15 @@ void frame_records(const Grid& g, int nx, int shot, int seg_tb, int seg, int4* o
     for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, 0, std::min(seg_tb, g.n2 - z0));
   for (int z0 = g.nz - g.n2; z0 < g.nz; z0 += seg_tb)
     for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, 1, std::min(seg_tb, g.nz - z0));
+  // side bands: both in one wave per row run (band 4) when each fits in half
+  // a wave with its inner neighbour and the halves do not overlap
+  const bool pack = g.n2 <= 30 && nx >= 64;
   for (int z0 = g.n2; z0 < g.nz - g.n2; z0 += seg) {
     const int rows = std::min(seg, g.nz - g.n2 - z0);
+    if (pack) {
+      put(z0, 0, 4, rows);
+      continue;
+    }
     for (int cx0 = -1; cx0 + 1 < g.n2; cx0 += 62) put(z0, cx0, 2, rows);
     for (int cx0 = nx - g.n2 - 1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, 3, rows);
   }
