@@ -723,6 +723,9 @@ def main():
                     help="rows per wave of the forward, reconstruction and receiver-step "
                          "kernels, comma-separated (instead of the autotuner; for profiles)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fuse-frame", action="store_true",
+                    help="run each forward and receiver step's interior and frame kernels as "
+                         "one launch (rtm_params.fuse_frame)")
     ap.add_argument("--host-gathers", action="store_true",
                     help="every timed step uploads the gathers from host memory (the PCIe-"
                          "inclusive rate of a caller that hands over host buffers)")
@@ -779,7 +782,8 @@ def main():
     ctx = hip.Context(nz=d.nz, nx=d.nx, n2=cfg.n2, nfdmax=cfg.nfdmax, nt=d.nt, nt2=d.nt2,
                       ilste=0, icompen=1, s_l=d.s_l, s_r=d.s_r, s_z=d.s_z, ds=1, n=cfg.n,
                       tao2=d.tao2, h2=d.h2, taoh=d.taoh, taoh2=d.taoh2, hzx2_1=d.hzx2_1,
-                      max_shots=a.shots, strip_format=strip_fmt, record=1, zseg=a.zseg)
+                      max_shots=a.shots, strip_format=strip_fmt, record=1, zseg=a.zseg,
+                      fuse_frame=1 if a.fuse_frame else 0)
     log(f"[bench] context: {ctx.device_bytes() / 2**30:.1f} GiB on device")
     mt, mm = P["models"]["true"], P["models"]["mig"]
     ctx.set_wavelet(P["wav"])
@@ -838,6 +842,12 @@ def main():
     upd_per_shot = steps_f * d.nz * d.nx + steps_f * (nI + d.nz * d.nx)
     value = upd_per_shot * a.shots * a.steps / el / 1e9
     kinds = {}
+    # With --fuse-frame a step's interior and frame work run as one launch
+    # (k_step): that launch is timed under the interior kind and moves the
+    # algorithmic bytes of both.
+    fusedk = {} if not a.fuse_frame else {
+        hip.K_FWD_INTERIOR: (hip.K_FWD_FRAME, "k_step<EPI_FWD+FR_FWD>"),
+        hip.K_RCVI_INTERIOR: (hip.K_RCV_FRAME, "k_step<EPI_RCVI+FR_RCV>")}
     for k, name in hip.KERNEL_NAMES.items():
         try:
             ms, n = ctx.kernel_time(k)
@@ -845,6 +855,9 @@ def main():
             continue
         if n:
             b = algorithmic_bytes(k, d, cfg.n2, cfg.nfdmax, a.shots, cfg.n)
+            if k in fusedk and ctx.kernel_time(fusedk[k][0])[1] == 0:
+                b += algorithmic_bytes(fusedk[k][0], d, cfg.n2, cfg.nfdmax, a.shots, cfg.n)
+                name = fusedk[k][1]
             kinds[name] = dict(total_ms=ms, launches=n, avg_ms=ms / n,
                                gbs=b / (ms / n * 1e-3) / 1e9, bytes_per_launch=b)
     dom = max(kinds, key=lambda k: kinds[k]["total_ms"])
@@ -871,7 +884,8 @@ def main():
                    "nt": d.nt, "shots": a.shots, "nfdmax": cfg.nfdmax, "n2": cfg.n2,
                    "strips": a.strips, "coefficients": "adaptive LS",
                    "velocity_bins": mm["nbins"], "rows_per_wave": zseg,
-                   "frame_runs": list(ctx.frame_runs())},
+                   "frame_runs": list(ctx.frame_runs()),
+                   "frame_launch": "one launch with the interior" if a.fuse_frame else "separate"},
         "shots_per_s": round(a.shots * a.steps / el, 3),
         "hbm_frac_whole_job": round(total_bytes / el / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(kd["gbs"], 1),

@@ -281,28 +281,41 @@ int watch_read(rtm_ctx* c, const char* pass) {
   return RTM_OK;
 }
 
+// Interior and frame of a step in one launch (k_step) when the context asks
+// for it and the pair is one of the fused kinds.
+static bool fused(const rtm_ctx* c, int epi, int mode) {
+  return c->p.fuse_frame != 0 && step_fusable(epi, mode);
+}
+
 int forward_step(rtm_ctx* c, int k) {
   const float wv = (k < c->p.nt2) ? c->wav[k - 1] : 0.0f;  // kernel.cu:812-813
   const bool quant = c->save_strips && c->st.fmt != RTM_STRIP_F32;
   if (quant && k >= 3) HIPCHK(c, launch_strip_finish(c->g, c->st, c->d_geo, k - 1, c->nshot, c->stream));
   {
+    // Add (FP32 combine) or Add_Con (FP64 combine) (kernel.cu:46-114) on the
+    // interior, Add + Hybrid1/2/3 on the frame (kernel.cu:116-208)
+    const int epi = c->p.ilste ? EPI_FWD64 : EPI_FWD;
     IntArgs a;
     fill_int_args(c, a);
     a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
     a.wav = wv; a.k = k;
-    Timed t(c, RTM_K_FWD_INTERIOR);
-    // Add (FP32 combine) or Add_Con (FP64 combine) (kernel.cu:46-114)
-    a.zs = zs_for(c, c->p.ilste ? EPI_FWD64 : EPI_FWD);
-    HIPCHK(c, launch_interior(c->p.ilste ? EPI_FWD64 : EPI_FWD, c->nfd, a, c->nshot, c->stream));
-  }
-  {
-    FrameArgs a;
-    fill_frame_args(c, a);
-    a.u1 = c->F[c->f1]; a.u0 = c->F[c->f0]; a.u2 = c->F[c->f2];
-    a.wav = wv; a.k = k;
-    a.save_strips = c->save_strips ? 1 : 0;
-    Timed t(c, RTM_K_FWD_FRAME);
-    HIPCHK(c, launch_frame(FR_FWD, c->nfd, a, c->nshot, c->ntiles, c->stream));
+    a.zs = zs_for(c, epi);
+    FrameArgs f;
+    fill_frame_args(c, f);
+    f.u1 = c->F[c->f1]; f.u0 = c->F[c->f0]; f.u2 = c->F[c->f2];
+    f.wav = wv; f.k = k;
+    f.save_strips = c->save_strips ? 1 : 0;
+    if (fused(c, epi, FR_FWD)) {
+      Timed t(c, RTM_K_FWD_INTERIOR);  // the fused launch counts as the interior kind
+      HIPCHK(c, launch_step(epi, FR_FWD, c->nfd, a, f, c->nshot, c->ntiles, c->stream));
+    } else {
+      {
+        Timed t(c, RTM_K_FWD_INTERIOR);
+        HIPCHK(c, launch_interior(epi, c->nfd, a, c->nshot, c->stream));
+      }
+      Timed t(c, RTM_K_FWD_FRAME);
+      HIPCHK(c, launch_frame(FR_FWD, c->nfd, f, c->nshot, c->ntiles, c->stream));
+    }
   }
   if (quant && k == c->p.nt - 1) HIPCHK(c, launch_strip_finish(c->g, c->st, c->d_geo, k, c->nshot, c->stream));
   if (c->save_strips && c->snap_slot[k] >= 0)  // checkpoint snapshot of u(k)
@@ -343,23 +356,30 @@ int backward_step(rtm_ctx* c, int k) {
   if (c->snap_slot[k] >= 0)  // checkpoint: the reconstructed u(k) is replaced by its snapshot
     HIPCHK(c, launch_snap_restore(c->g, c->F[c->f2], c->sn, c->d_geo, c->snap_slot[k], c->nshot, c->stream));
   { int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t; }  // Deliver_EFF (kernel.cu:903)
-  {  // BKAdd (kernel.cu:913) + Rel_* (kernel.cu:923-930)
+  {  // BKAdd (kernel.cu:913) + Rel_* (kernel.cu:923-930) on the interior, BKAdd
+     // on the frame + BKHybrid1/2 (kernel.cu:913-920)
     IntArgs a;
     fill_int_args(c, a);
     a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
     a.uk = c->F[c->f1];
     a.wav = 0.f; a.k = k;
-    Timed t(c, split ? RTM_K_RCVI_INTERIOR : RTM_K_RCV_INTERIOR);
     a.zs = zs_for(c, ercv);
-    HIPCHK(c, launch_interior(ercv, c->nfd, a, c->nshot, c->stream));
-  }
-  {  // BKAdd on the frame + BKHybrid1/2 (kernel.cu:913-920)
-    FrameArgs a;
-    fill_frame_args(c, a);
-    a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
-    a.wav = 0.f; a.k = k;
-    Timed t(c, RTM_K_RCV_FRAME);
-    HIPCHK(c, launch_frame(FR_RCV, c->nfd, a, c->nshot, c->ntiles, c->stream));
+    FrameArgs f;
+    fill_frame_args(c, f);
+    f.u1 = c->Q[c->q1]; f.u0 = c->Q[c->q0]; f.u2 = c->Q[c->q2];
+    f.wav = 0.f; f.k = k;
+    const int kind = split ? RTM_K_RCVI_INTERIOR : RTM_K_RCV_INTERIOR;
+    if (fused(c, ercv, FR_RCV)) {
+      Timed t(c, kind);  // the fused launch counts as the interior kind
+      HIPCHK(c, launch_step(ercv, FR_RCV, c->nfd, a, f, c->nshot, c->ntiles, c->stream));
+    } else {
+      {
+        Timed t(c, kind);
+        HIPCHK(c, launch_interior(ercv, c->nfd, a, c->nshot, c->stream));
+      }
+      Timed t(c, RTM_K_RCV_FRAME);
+      HIPCHK(c, launch_frame(FR_RCV, c->nfd, f, c->nshot, c->ntiles, c->stream));
+    }
   }
   if (watch_due(c, k, 0)) {  // the reconstructed u(k) and the new receiver field
     int rc = watch(c, c->F[c->f1], k);
@@ -1484,7 +1504,11 @@ int rtm_autotune(rtm_ctx* c, int nsteps, int* zseg) {
     const int seg0 = frame_segment_rows(g.nfd);
     const int cand[2][3] = {{g.n2, (g.n2 + 1) / 2, (g.n2 + 2) / 3},
                             {seg0, std::max(1, seg0 / 2), 2 * seg0}};
-    const unsigned fmask = (1u << RTM_K_FWD_FRAME) | (1u << RTM_K_RCV_FRAME);
+    // the frame kernels' time: their own kinds, or the fused launches that
+    // carry them (counted under the interior kinds)
+    const unsigned fmask =
+        (fused(c, fwd, FR_FWD) ? 1u << RTM_K_FWD_INTERIOR : 1u << RTM_K_FWD_FRAME) |
+        (fused(c, kinds[2], FR_RCV) ? 1u << timers[2] : 1u << RTM_K_RCV_FRAME);
     for (int which = 0; which < 2 && rc == RTM_OK; ++which) {
       int& run = which == 0 ? c->fseg_tb : c->fseg_side;
       double tot[3] = {0.0, 0.0, 0.0};

@@ -135,6 +135,12 @@ struct FrameArgs {
 // multiple of it.
 int interior_queue_rows(int epi, int nfd);
 hipError_t launch_interior(int epi, int nfd_inst, IntArgs a, int nshot, hipStream_t s);
+// One launch of a time step's interior kernel (epi) and frame kernel (mode),
+// their blocks interleaved (k_step); only (EPI_FWD, FR_FWD) and
+// (EPI_RCVI, FR_RCV) are fused (step_fusable).
+bool step_fusable(int epi, int mode);
+hipError_t launch_step(int epi, int mode, int nfd, IntArgs ia, FrameArgs fa, int nshot, int nrec,
+                       hipStream_t s);
 hipError_t launch_frame(int mode, int nfd_inst, FrameArgs a, int nshot, int nrecords,
                         hipStream_t s);
 // Frame records of one shot of grid width nx (k_frame): the top and bottom

@@ -87,6 +87,9 @@ __device__ __forceinline__ void bst4s(Rsrc r, unsigned vo, unsigned so, float a,
   const u32x4 v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c), __float_as_uint(d)};
   __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)(vo + so), 0, RTM_STREAM_POLICY);
 }
+__device__ __forceinline__ void bst1s_(Rsrc r, unsigned vo, unsigned so, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(vo + so), 0, RTM_STREAM_POLICY);
+}
 
 template <int NFD>
 struct CoefRow {
@@ -269,8 +272,10 @@ struct EpiTraits {
   static constexpr int PD = (rcv || born || adj) ? 1 : 2;  // prefetch distance of the stencil operands
 };
 
+// Interior body for block bid (k_interior: bid = blockIdx.x; k_step gives
+// the interior blocks of a launch shared with the frame their own numbering).
 template <int NFD, int EPI>
-__global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
+__device__ __forceinline__ void interior_body(const IntArgs& a, const unsigned bid) {
   using T = EpiTraits<EPI>;
   constexpr int H = (NFD + 3) & ~3;
   constexpr int PD = T::PD;
@@ -297,13 +302,13 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   int tile;
   if (a.nshot >= 4) {
     const unsigned nsg = ((unsigned)a.nshot + 3u) >> 2;
-    shot = (int)(blockIdx.x % nsg) * 4 + wv;
-    tile = (int)(blockIdx.x / nsg);
+    shot = (int)(bid % nsg) * 4 + wv;
+    tile = (int)(bid / nsg);
     if (shot >= a.nshot) return;
     zseg = tile / a.nxt;
   } else {
-    shot = (int)(blockIdx.x % (unsigned)a.nshot);
-    tile = (int)(blockIdx.x / (unsigned)a.nshot);
+    shot = (int)(bid % (unsigned)a.nshot);
+    tile = (int)(bid / (unsigned)a.nshot);
     zseg = (tile / a.nxt) * 4 + wv;
   }
   const int xt = tile % a.nxt;
@@ -322,6 +327,14 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   bool colok[4];
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) colok[jj] = c0 + jj < cend;
+  // The last four columns of an interior row may reach 1..3 frame columns
+  // (interior widths that are not multiples of 4).  Those belong to the frame
+  // kernel, which may run in the same launch (k_step), so the lane holding
+  // them stores its valid columns one by one; only the wave whose tile holds
+  // the interior's right edge (uniform) issues those stores.
+  const bool edge_tile = cend < cb + 256;
+  const bool part = !colok[3];
+  const unsigned vst4 = part ? kDrop : vst;
 
   // Halo columns: the first 2H/4 lanes fetch them; the others fetch nothing
   // (a dropped load returns 0) and write that to their own slot, which the
@@ -507,7 +520,12 @@ __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
       }
       const unsigned rowdrop = zz < zhi ? 0u : kDrop;
       const unsigned rz = rowb(zz);
-      bst4s(R2, vst | rowdrop, rz, out[0], out[1], out[2], out[3]);
+      bst4s(R2, vst4 | rowdrop, rz, out[0], out[1], out[2], out[3]);
+      if (edge_tile) {
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+          bst1s_(R2, ((part && colok[jj]) ? vst + 4u * jj : kDrop) | rowdrop, rz, out[jj]);
+      }
       if (T::rcv) {  // imaging (kernel.cu:489-517); frame columns keep their values
         const float4 ukv = ukb[0], i1 = i1b[0], i2 = i2b[0], su = sub[0], sq = sqb[0];
         float r1o[4], r2o[4], rsu[4], rsq[4];
@@ -617,7 +635,7 @@ __device__ __forceinline__ int strip_elem(const Grid& g, int nx, int mnx, int z,
 }
 
 template <int NFD, int MODE>
-__global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
+__device__ __forceinline__ void frame_body(const FrameArgs& a, const unsigned bid) {
   constexpr int PD = 2;                   // rows of prefetch
   constexpr int NB = NFD > 2 ? NFD : 2;   // queue rows kept behind the T row
   // The row loop is unrolled by U, a multiple of 12 that holds the u1 rows
@@ -632,7 +650,7 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
   const Model& m = a.m;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ri = (int)blockIdx.x * 4 + wv;
+  const int ri = (int)bid * 4 + wv;
   if (ri >= a.nrec) return;
   const int4 rec = a.recs[ri];
   const int z0 = rec.x, cx0 = rec.y, band = rec.z & 15, nrows = rec.z >> 4, shot = rec.w;
@@ -938,6 +956,40 @@ __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
         }
       }
     }
+  }
+}
+
+template <int NFD, int EPI>
+__global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
+  interior_body<NFD, EPI>(a, blockIdx.x);
+}
+
+template <int NFD, int MODE>
+__global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
+  frame_body<NFD, MODE>(a, blockIdx.x);
+}
+
+// One time step's interior and frame work in one launch.  The two write
+// disjoint cells of u(k+1) and only read u(k), u(k-1) and the model, so they
+// may run concurrently; the interior kernel waits on memory most of the time
+// and the frame kernel on instruction issue, so frame blocks placed among the
+// interior blocks fill the interior's idle issue slots.  Blocks go in units
+// of 8 (one per XCD): every R-th unit is a frame unit while frame units last,
+// so an interior block keeps its own index modulo 8 (its XCD, which its L2
+// sharing of model rows relies on); indices past a part's block count exit.
+template <int NFD, int EPI, int MODE>
+__global__ __launch_bounds__(256) void k_step(const IntArgs ia, const FrameArgs fa, unsigned nint,
+                                              unsigned nfr) {
+  const unsigned nfu = (nfr + 7u) >> 3, niu = (nint + 7u) >> 3;
+  const unsigned R = (nfu + niu) / nfu;
+  const unsigned u = blockIdx.x >> 3, s8 = blockIdx.x & 7u;
+  const unsigned before = min(nfu, (u + R - 1u) / R);  // frame units ahead of unit u
+  if (u % R == 0u && u / R < nfu) {
+    const unsigned b = (u / R) * 8u + s8;
+    if (b < nfr) frame_body<NFD, MODE>(fa, b);
+  } else {
+    const unsigned b = (u - before) * 8u + s8;
+    if (b < nint) interior_body<NFD, EPI>(ia, b);
   }
 }
 
@@ -1417,6 +1469,43 @@ hipError_t launch_frame(int mode, int nfd, FrameArgs a, int nshot, int nrec, hip
     else if (mode == FR_BORN) hipLaunchKernelGGL((k_frame<N, FR_BORN>), grid, block, 0, s, a); \
     else if (mode == FR_ADJ) hipLaunchKernelGGL((k_frame<N, FR_ADJ>), grid, block, 0, s, a); \
     else hipLaunchKernelGGL((k_frame<N, FR_RCV>), grid, block, 0, s, a);                 \
+    break;
+  switch (nfd) {
+    RTM_NFD_CASES(CASE)
+    default: return hipErrorInvalidValue;
+  }
+#undef CASE
+  return hipGetLastError();
+}
+
+// The interior grid of launch_interior (rows per wave rounded to the queue
+// length) and one wave per frame record, in one k_step launch; only the
+// forward and the split receiver step are fused (the other kinds keep two
+// launches).
+bool step_fusable(int epi, int mode) {
+  return (epi == EPI_FWD && mode == FR_FWD) || (epi == EPI_RCVI && mode == FR_RCV);
+}
+
+hipError_t launch_step(int epi, int mode, int nfd, IntArgs ia, FrameArgs fa, int nshot, int nrec,
+                       hipStream_t s) {
+  if (!step_fusable(epi, mode)) return hipErrorInvalidValue;
+  const int ql = interior_queue_rows(epi, nfd);
+  const int mult = ia.zs > ql ? (ia.zs + ql / 2) / ql : 1;
+  ia.zs = mult * ql;
+  const int nseg = (ia.g.mnz + ia.zs - 1) / ia.zs;
+  ia.nshot = nshot;
+  const unsigned nint = (unsigned)(nshot >= 4 ? ia.nxt * nseg * ((nshot + 3) / 4)
+                                              : ia.nxt * ((nseg + 3) / 4) * nshot);
+  fa.nshot = nshot;
+  fa.nrec = nrec;
+  const unsigned nfr = (unsigned)((nrec + 3) / 4);
+  if (nfr == 0 || nint == 0) return hipErrorInvalidValue;
+  const unsigned nfu = (nfr + 7u) / 8u, niu = (nint + 7u) / 8u;
+  dim3 grid((nfu + niu) * 8u), block(256);
+#define CASE(N)                                                                               \
+  case N:                                                                                     \
+    if (epi == EPI_FWD) hipLaunchKernelGGL((k_step<N, EPI_FWD, FR_FWD>), grid, block, 0, s, ia, fa, nint, nfr); \
+    else hipLaunchKernelGGL((k_step<N, EPI_RCVI, FR_RCV>), grid, block, 0, s, ia, fa, nint, nfr); \
     break;
   switch (nfd) {
     RTM_NFD_CASES(CASE)

@@ -32,7 +32,8 @@ class RtmParams(C.Structure):
                                         "icompen", "s_l", "s_r", "s_z", "ds", "n")] + \
                [(n, C.c_float) for n in ("tao2", "h2", "taoh", "taoh2", "hzx2_1")] + \
                [(n, C.c_int) for n in ("max_shots", "strip_format", "record", "zseg",
-                                       "snap_every", "stack_nx", "check_every")]
+                                       "snap_every", "stack_nx", "check_every",
+                                       "fuse_frame")]
 
 
 class RtmShotGeom(C.Structure):

@@ -292,6 +292,41 @@ def test_frame_runs_keep_results():
         hip_context(case, nshot=1).set_frame_runs(n2 + 1, 20)
 
 
+def test_fused_step_launch_keeps_results():
+    """A step's interior and frame work in one launch (fuse_frame) gives the
+    fields, strips and images of two separate launches bit for bit, for
+    several shots (interior blocks of four shots) and for one shot (four row
+    segments per block), with the frame runs shortened so that frame units
+    spread among the interior units."""
+    from rtm_gpu_amd import hip
+    for nshot in (3, 5, 1):
+        case = build_case(name=f"fuse{nshot}", nshot=nshot, depth_step=20.0)
+        seis = np.stack([synth_gather(case, s) for s in range(nshot)])
+        keys = ("nz", "nx", "n2", "nfdmax", "nt", "nt2", "ilste", "icompen", "s_l", "s_r",
+                "s_z", "ds", "n", "tao2", "h2", "taoh", "taoh2", "hzx2_1")
+        out = []
+        for fuse in (0, 1):
+            cf, m = case["cfg"], case["model"]
+            ctx = hip.Context(**{k: cf[k] for k in keys}, max_shots=nshot, fuse_frame=fuse)
+            ctx.set_model(m["v"], m["r1"], m["w"], m["c"], m["index"], m["nvel"], m["vmin"], m["dv"])
+            ctx.set_wavelet(m["wav"])
+            ctx.set_shots(case["r_u"], case["r_x"])
+            ctx.set_frame_runs((cf["n2"] + 2) // 3, 10)
+            ctx.set_gathers(seis)
+            ctx.forward_begin()
+            ctx.forward_advance(cf["nt"] // 2)
+            fwd = [ctx.field(hip.FIELD_U, shot=s) for s in range(nshot)]
+            strip = ctx.strip(nshot - 1, cf["nt"] // 2 - 1)
+            ctx.migrate()
+            out.append((fwd, strip, [ctx.field(w, shot=s) for s in range(nshot)
+                                     for w in (hip.FIELD_I1, hip.FIELD_I2, hip.FIELD_Q)]))
+            ctx.close()
+        (f0, s0, i0), (f1, s1, i1) = out
+        assert all(np.array_equal(a, b) for a, b in zip(f0, f1)), nshot
+        assert np.array_equal(s0, s1), nshot
+        assert all(np.array_equal(a, b) for a, b in zip(i0, i1)), nshot
+
+
 def test_autotune_keeps_results():
     """The autotuner only changes the row-segment height of the interior
     kernels and the frame kernels' row runs; images after tuning equal the
