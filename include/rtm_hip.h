@@ -280,9 +280,10 @@ int rtm_autotune(rtm_ctx* ctx, int nsteps, int* zseg);
  * adjoint take the receiver step's). */
 int rtm_set_zseg(rtm_ctx* ctx, const int* zseg);
 /* Rows per frame-kernel wave: the top and bottom bands run in pieces of
- * rows_tb rows (ceil(n2/3)..n2), the side bands in pieces of rows_side rows
- * (from half the default up to nz).  rtm_autotune picks both; rtm_frame_runs
- * reads the current pair. */
+ * rows_tb rows (ceil(n2/3)..n2; where the four-column kernel takes the middle
+ * of those bands, rows_tb applies to their ends), the side bands in pieces of
+ * rows_side rows (from a quarter of the default up to nz).  rtm_autotune picks
+ * both; rtm_frame_runs reads the current pair. */
 int rtm_set_frame_runs(rtm_ctx* ctx, int rows_tb, int rows_side);
 int rtm_frame_runs(rtm_ctx* ctx, int* rows_tb, int* rows_side);
 

@@ -55,7 +55,8 @@ struct rtm_ctx {
   std::vector<ShotGeo> geo;
   int gw = 0;                  // columns of the stacked image
   int4* d_tiles = nullptr;     // frame (tile, shot) records of the batch
-  int ntiles = 0, tile_cap = 0;
+  int ntiles = 0, tile_cap = 0;  // FRAME_ALL records; capacity of d_tiles
+  int nsplit = 0, nvec = 0;      // FRAME_SCALAR and FRAME_VEC records after them
   int fseg_tb = 0, fseg_side = 0;  // rows per frame record: top/bottom bands, side bands
   // model windows of a ragged batch (allocated on first use): slot w holds
   // one distinct window; shots whose windows are equal share a slot
@@ -287,6 +288,22 @@ static bool fused(const rtm_ctx* c, int epi, int mode) {
   return c->p.fuse_frame != 0 && step_fusable(epi, mode);
 }
 
+// The frame of a forward or receiver step: split between k_frame (the band
+// ends at the corners, the side bands) and k_frame4 (the edge cells of the top
+// and bottom bands) when the batch has float4 records, else k_frame on the
+// FRAME_ALL records.  The two launches write disjoint cells.
+hipError_t launch_frame_split(rtm_ctx* c, int mode, FrameArgs f) {
+  if (c->nvec > 0 && frame4_supported(c->nfd)) {
+    FrameArgs fs = f;
+    fs.recs = c->d_tiles + c->ntiles;
+    const hipError_t e = launch_frame(mode, c->nfd, fs, c->nshot, c->nsplit, c->stream);
+    if (e != hipSuccess) return e;
+    fs.recs = c->d_tiles + c->ntiles + c->nsplit;
+    return launch_frame4(mode, c->nfd, fs, c->nshot, c->nvec, c->stream);
+  }
+  return launch_frame(mode, c->nfd, f, c->nshot, c->ntiles, c->stream);
+}
+
 int forward_step(rtm_ctx* c, int k) {
   const float wv = (k < c->p.nt2) ? c->wav[k - 1] : 0.0f;  // kernel.cu:812-813
   const bool quant = c->save_strips && c->st.fmt != RTM_STRIP_F32;
@@ -314,7 +331,7 @@ int forward_step(rtm_ctx* c, int k) {
         HIPCHK(c, launch_interior(epi, c->nfd, a, c->nshot, c->stream));
       }
       Timed t(c, RTM_K_FWD_FRAME);
-      HIPCHK(c, launch_frame(FR_FWD, c->nfd, f, c->nshot, c->ntiles, c->stream));
+      HIPCHK(c, launch_frame_split(c, FR_FWD, f));
     }
   }
   if (quant && k == c->p.nt - 1) HIPCHK(c, launch_strip_finish(c->g, c->st, c->d_geo, k, c->nshot, c->stream));
@@ -378,7 +395,7 @@ int backward_step(rtm_ctx* c, int k) {
         HIPCHK(c, launch_interior(ercv, c->nfd, a, c->nshot, c->stream));
       }
       Timed t(c, RTM_K_RCV_FRAME);
-      HIPCHK(c, launch_frame(FR_RCV, c->nfd, f, c->nshot, c->ntiles, c->stream));
+      HIPCHK(c, launch_frame_split(c, FR_RCV, f));
     }
   }
   if (watch_due(c, k, 0)) {  // the reconstructed u(k) and the new receiver field
@@ -482,38 +499,50 @@ int adjoint_step(rtm_ctx* c, int k) {
   return adjoint_update(c, k, true);
 }
 
-// Frame records (k_frame) of a grid nx columns wide, tagged with the shot:
-// top and bottom bands in runs of seg_tb rows, side bands in runs of seg.
-void build_tiles(const Grid& g, int nx, int shot, int seg_tb, int seg, std::vector<int4>& t) {
+// Frame records of a grid nx columns wide, tagged with the shot: top and
+// bottom bands in runs of seg_tb rows, side bands in runs of seg (kind:
+// FRAME_ALL, FRAME_SCALAR or FRAME_VEC).
+void build_tiles(const Grid& g, int nx, int shot, int seg_tb, int seg, int kind,
+                 std::vector<int4>& t) {
   int n = 0;
-  frame_records(g, nx, shot, seg_tb, seg, nullptr, &n);
+  frame_records(g, nx, shot, seg_tb, seg, kind, nullptr, &n);
   const size_t at = t.size();
   t.resize(at + n);
-  frame_records(g, nx, shot, seg_tb, seg, t.data() + at, &n);
+  frame_records(g, nx, shot, seg_tb, seg, kind, t.data() + at, &n);
 }
 
 int choose_zs(rtm_ctx* c, int nshot);
 void drop_graph(rtm_ctx* c);
 
-// The batch's frame records for the context's run lengths: tile-major with
-// the shot fastest, so that one XCD holds many shots of the same tiles at a
-// time.
+// The batch's frame records for the context's run lengths, three lists one
+// after the other: every frame cell for k_frame (FRAME_ALL: the Born and
+// adjoint frames, the one-launch step), then the split frame of the forward
+// and receiver fields (FRAME_SCALAR for k_frame, FRAME_VEC for k_frame4).
+// Each list is tile-major with the shot fastest, so that one XCD holds many
+// shots of the same tiles at a time.
 int upload_frame_records(rtm_ctx* c) {
   const int ns = (int)c->geo.size();
-  std::vector<std::vector<int4>> per(ns);
-  size_t most = 0;
-  for (int s = 0; s < ns; ++s) {
-    build_tiles(c->g, c->geo[s].nx, s, c->fseg_tb, c->fseg_side, per[s]);
-    most = std::max(most, per[s].size());
-  }
   std::vector<int4> rec;
-  for (size_t t = 0; t < most; ++t)
-    for (int s = 0; s < ns; ++s)
-      if (t < per[s].size()) rec.push_back(per[s][t]);
-  if ((int)rec.size() > c->tile_cap) return fail(c, RTM_E_ARG, "frame records exceed capacity");
+  int count[3] = {0, 0, 0};
+  for (int kind = FRAME_ALL; kind <= FRAME_VEC; ++kind) {
+    std::vector<std::vector<int4>> per(ns);
+    size_t most = 0;
+    for (int s = 0; s < ns; ++s) {
+      build_tiles(c->g, c->geo[s].nx, s, c->fseg_tb, c->fseg_side, kind, per[s]);
+      most = std::max(most, per[s].size());
+    }
+    const size_t at = rec.size();
+    for (size_t t = 0; t < most; ++t)
+      for (int s = 0; s < ns; ++s)
+        if (t < per[s].size()) rec.push_back(per[s][t]);
+    count[kind] = (int)(rec.size() - at);
+  }
+  if ((long long)rec.size() > c->tile_cap) return fail(c, RTM_E_ARG, "frame records exceed capacity");
   HIPCHK(c, hipMemcpyAsync(c->d_tiles, rec.data(), rec.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->ntiles = (int)rec.size();
+  c->ntiles = count[FRAME_ALL];
+  c->nsplit = count[FRAME_SCALAR];
+  c->nvec = count[FRAME_VEC];
   drop_graph(c);
   return RTM_OK;
 }
@@ -706,9 +735,14 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
     // the widest grid with the shortest runs the autotuner tries has the most records
     c->fseg_tb = g.n2;
     c->fseg_side = frame_segment_rows(g.nfd);
+    // (a split frame of a narrower shot may take a few records more than
+    // its FRAME_ALL list: at most two end records and one float4 record per
+    // band run beyond it)
+    const int min_tb = std::max(1, (g.n2 + 2) / 3);
     std::vector<int4> tiles;
-    build_tiles(g, g.nx, 0, std::max(1, (g.n2 + 2) / 3), std::max(1, c->fseg_side / 2), tiles);
-    c->tile_cap = (int)tiles.size() * p->max_shots;
+    build_tiles(g, g.nx, 0, min_tb, std::max(1, c->fseg_side / 4), FRAME_ALL, tiles);
+    const int runs = 2 * ((g.n2 + min_tb - 1) / min_tb);
+    c->tile_cap = (2 * (int)tiles.size() + runs * (2 * ((g.n2 + 62) / 62) + 1)) * p->max_shots;
   }
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_tiles, (size_t)c->tile_cap);
   if (rc != RTM_OK) {
@@ -922,7 +956,7 @@ int rtm_set_shot_model(rtm_ctx* c, int shot, const float* v, const float* r1) {
     c->win_fp[2 * w] = fp[0];
     c->win_fp[2 * w + 1] = fp[1];
     c->tp_ok = false;
-  }
+    }
   c->win_ref[w] += 1;
   c->geo[shot].win = w;
   HIPCHK(c, hipMemcpyAsync(c->d_geo + shot, &c->geo[shot], sizeof(ShotGeo), hipMemcpyHostToDevice, c->stream));
@@ -1401,7 +1435,7 @@ int rtm_set_zseg(rtm_ctx* c, const int* zseg) {
 int rtm_set_frame_runs(rtm_ctx* c, int rows_tb, int rows_side) {
   if (!c) return RTM_E_ARG;
   const Grid& g = c->g;
-  const int min_tb = std::max(1, (g.n2 + 2) / 3), min_side = std::max(1, frame_segment_rows(g.nfd) / 2);
+  const int min_tb = std::max(1, (g.n2 + 2) / 3), min_side = std::max(1, frame_segment_rows(g.nfd) / 4);
   if (rows_tb < min_tb || rows_tb > g.n2 || rows_side < min_side || rows_side > g.nz)
     return fail(c, RTM_E_ARG, "frame runs %d / %d outside %d..%d / %d..%d", rows_tb, rows_side,
                 min_tb, g.n2, min_side, g.nz);
@@ -1496,14 +1530,15 @@ int rtm_autotune(rtm_ctx* c, int nsteps, int* zseg) {
     c->zs_epi[epi] = kMult[bi] * ql;
   }
   // Frame record runs, timed on both frame kernels of the step pair: the top
-  // and bottom bands in runs of n2, n2/2 or n2/3 rows, then the side bands in
-  // runs of half, once or twice the default.  Shorter runs put more waves in
-  // flight at the cost of recomputing two pre-blend rows per run.
+  // and bottom bands in runs of n2, n2/2 or n2/3 rows (with float4 records:
+  // their k_frame ends), then the side bands in runs of the default, half or a
+  // quarter of it.  Shorter runs put more waves in flight at the cost of
+  // recomputing two pre-blend rows per run.
   if (rc == RTM_OK) {
     const Grid& g = c->g;
     const int seg0 = frame_segment_rows(g.nfd);
     const int cand[2][3] = {{g.n2, (g.n2 + 1) / 2, (g.n2 + 2) / 3},
-                            {seg0, std::max(1, seg0 / 2), 2 * seg0}};
+                            {seg0, std::max(1, seg0 / 2), std::max(1, seg0 / 4)}};
     // the frame kernels' time: their own kinds, or the fused launches that
     // carry them (counted under the interior kinds)
     const unsigned fmask =

@@ -112,8 +112,10 @@ struct FrameArgs {
   const float* u1;
   const float* u0;
   float* u2;
-  // one record per wave: (first output row, column of lane 0, band | rows << 4,
-  // shot); band 0 top, 1 bottom, 2 left, 3 right, 4 both sides in one wave
+  // one record per wave: (first output row, column of lane 0, band | rows << 4
+  // | ncol << 20, shot); band 0 top, 1 bottom, 2 left, 3 right, 4 both sides
+  // in one wave; ncol (top/bottom) limits the written lanes to 1..ncol (0: 62).
+  // k_frame4 records: (first output row, first column of the 256, band, shot).
   // (frame_records)
   const int4* recs;
   const ShotGeo* geo;
@@ -143,12 +145,22 @@ hipError_t launch_step(int epi, int mode, int nfd, IntArgs ia, FrameArgs fa, int
                        hipStream_t s);
 hipError_t launch_frame(int mode, int nfd_inst, FrameArgs a, int nshot, int nrecords,
                         hipStream_t s);
-// Frame records of one shot of grid width nx (k_frame): the top and bottom
-// bands in runs of 64 columns (62 written), the left and right bands in runs
-// of 64 columns by `seg` rows.
-// Frame records of one shot: top and bottom bands in runs of seg_tb rows,
-// the side bands in runs of seg rows; 62 written columns per record.
-void frame_records(const Grid& g, int nx, int shot, int seg_tb, int seg, int4* out, int* count);
+// Frame records of one shot of grid width nx: top and bottom bands in runs of
+// seg_tb rows, the side bands in runs of seg rows.  FRAME_ALL: every frame
+// cell for k_frame (62 written columns per record); FRAME_SCALAR: the k_frame
+// part of a split frame (the n2+1 columns at each end of the top and bottom
+// bands, and the side bands); FRAME_VEC: the k_frame4 part (the edge cells of
+// the top and bottom bands, 252 columns per record).  A split frame is every
+// frame cell once; when the shot's bands hold no float4 record, FRAME_SCALAR
+// equals FRAME_ALL and FRAME_VEC is empty.
+enum { FRAME_ALL = 0, FRAME_SCALAR = 1, FRAME_VEC = 2 };
+void frame_records(const Grid& g, int nx, int shot, int seg_tb, int seg, int kind, int4* out,
+                   int* count);
+// The float4 top/bottom frame kernel (forward and receiver fields) exists for
+// these compile-time stencil half-lengths.
+bool frame4_supported(int nfd_inst);
+hipError_t launch_frame4(int mode, int nfd_inst, FrameArgs a, int nshot, int nrecords,
+                         hipStream_t s);
 int frame_segment_rows(int nfd);
 hipError_t launch_fill(float* p, long long n, float val, hipStream_t s);
 // Neighbour-tap planes (Model::tp) of one model window from its coefficient

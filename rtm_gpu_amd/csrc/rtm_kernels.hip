@@ -653,7 +653,8 @@ __device__ __forceinline__ void frame_body(const FrameArgs& a, const unsigned bi
   const int ri = (int)bid * 4 + wv;
   if (ri >= a.nrec) return;
   const int4 rec = a.recs[ri];
-  const int z0 = rec.x, cx0 = rec.y, band = rec.z & 15, nrows = rec.z >> 4, shot = rec.w;
+  const int z0 = rec.x, cx0 = rec.y, band = rec.z & 15, nrows = (rec.z >> 4) & 0xffff, shot = rec.w;
+  const int ncol = (rec.z >> 20) & 0xff;  // top/bottom: written lanes 1..ncol (0: 1..62)
   const ShotGeo sg = a.geo[shot];
   const int NX = sg.nx, NZ = g.nz, N2 = g.n2;
   // Band 4 packs both side bands of a row run into one wave (n2 <= 30):
@@ -669,7 +670,7 @@ __device__ __forceinline__ void frame_body(const FrameArgs& a, const unsigned bi
   else if (band == 3) ccl = NX - N2;
   else if (packed) { ccl = rhalf ? NX - N2 : 0; ccr = rhalf ? NX : N2; }
   const int hl_ = packed ? (lane & 31) : lane;  // lane within its (half) run
-  const bool outcol = hl_ >= 1 && hl_ <= (packed ? 30 : 62) && x >= ccl && x < ccr;
+  const bool outcol = hl_ >= 1 && hl_ <= (packed ? 30 : (ncol ? ncol : 62)) && x >= ccl && x < ccr;
   // LDS slot of the lane's column (packed: the right half after the left
   // half's right halo and its own left halo)
   const int slot = packed && rhalf ? 3 * NFD + lane : NFD + lane;
@@ -959,9 +960,253 @@ __device__ __forceinline__ void frame_body(const FrameArgs& a, const unsigned bi
   }
 }
 
+// ---------------------------------------------------------------------------
+// Top and bottom bands away from the corners, four columns per lane.  A record
+// (z0, cs, band | rows << 4, shot) covers the 256 columns cs .. cs+255 (cs ==
+// n2 mod 4, so each lane's float4 is aligned like the interior kernel's) and
+// writes the columns cs+1 .. cs+252 left of nx-n2-1.  Every written cell is a
+// top-edge (bottom-edge) cell of Hybrid1/Hybrid2 (kernel.cu:122-125, 130-133):
+// its one-way update reads the inner row and the lateral neighbours x-1, x+1
+// of the same row, so lane 0's first column and lane 63's last three only
+// supply neighbours.  The row march, the look-ahead of the operands and the
+// arithmetic are those of frame_body's top/bottom path, operation for
+// operation, so both give the same bits; per cell this one spends a quarter of
+// the row addressing, masks and buffer instructions.  FR_FWD and FR_RCV only.
+template <int NFD, int MODE, bool BOT>
+__device__ __forceinline__ void frame_tb4_body(const FrameArgs& a, const int4 rec) {
+  static_assert(MODE == FR_FWD || MODE == FR_RCV, "forward and receiver frames only");
+  constexpr int PD = 2;                   // rows of prefetch
+  constexpr int NB = NFD > 2 ? NFD : 2;   // queue rows kept behind the T row
+  constexpr int U = (NB + NFD + 1 + PD + 11) / 12 * 12;
+  constexpr int H = (NFD + 3) & ~3;
+  constexpr int W4 = 64 + 2 * (H / 4);
+  constexpr int NWIN = 1 + 2 * (H / 4);
+  __shared__ float4 lds[4][W4];
+  const Grid& g = a.g;
+  const Model& m = a.m;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int z0 = rec.x, cs = rec.y, nrows = (rec.z >> 4) & 0xffff, shot = rec.w;
+  const ShotGeo sg = a.geo[shot];
+  const int NX = sg.nx, NZ = g.nz, N2 = g.n2;
+  const int wend = NX - N2 - 1;  // first column past the edge cells
+  const int c0 = cs + 4 * lane;
+  const int clast = NX - ((NX - N2) & 3);  // largest c == n2 (mod 4) with c <= nx
+  const unsigned vcol = (unsigned)(g.x0 + min(c0, clast)) * 4u;
+  bool wr[4];
+  bool full = true;
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    wr[jj] = c0 + jj > cs && c0 + jj <= cs + 252 && c0 + jj < wend;
+    full = full && wr[jj];
+  }
+  const unsigned vst = (unsigned)(g.x0 + c0) * 4u;
+  // halo float4s of the LDS row: lanes 0 .. H/4-1 left, H/4 .. 2H/4-1 right
+  // (columns past the shot's width only feed unwritten columns)
+  unsigned vhal = kDrop;
+  int hpos = H / 4 + lane;
+  if (lane < H / 4) { vhal = (unsigned)(g.x0 + cs - H + 4 * lane) * 4u; hpos = lane; }
+  else if (lane < 2 * (H / 4)) {
+    vhal = (unsigned)(g.x0 + min(cs + 256 + 4 * (lane - H / 4), clast)) * 4u;
+    hpos = 64 + lane;
+  }
+
+  const long long so = (long long)shot * g.fstride;
+  const unsigned fb = (unsigned)g.fstride * 4u;
+  const Rsrc R1 = rsrc(a.u1 + so, fb), R0 = rsrc(a.u0 + so, fb), R2 = rsrc(a.u2 + so, fb);
+  const long long mo = (long long)sg.win * m.mstride;
+  const Rsrc RP = rsrc(m.pm + mo, fb), RK = rsrc(m.ko + mo, fb), RV = rsrc(m.v + mo, fb);
+  const Rsrc RC = rsrc(m.ctab, (unsigned)(m.nrow * m.cs) * 4u);
+  const Rsrc RW = rsrc(m.w, (unsigned)(N2 + 1) * 4u);
+  const unsigned P4 = (unsigned)g.pitch * 4u;
+  auto rowm = [&](int z) -> unsigned { return (unsigned)mirror_idx(z, NZ) * P4; };
+  auto rowc = [&](int z) -> unsigned { return (unsigned)min(max(z, 0), NZ - 1) * P4; };
+  // ring of row zr: zr (top edge), nz-1-zr (bottom edge)
+  auto ring = [&](int zr) -> int { return BOT ? NZ - 1 - zr : zr; };
+
+  // receiver field: this step's data sample (k+1) of each receiver column,
+  // replacing the field on row s_z where non-zero (kernel.cu:349-354)
+  // (a column without a receiver reads 0: its dropped load returns 0)
+  float ev[4] = {0.f, 0.f, 0.f, 0.f};
+  if (MODE == FR_RCV) {
+    Rcv rr = a.r;
+    rr.s_l = sg.s_l;
+    rr.s_r = sg.s_r;
+    const Rsrc RD = rsrc(a.data + ((long long)shot * a.nt + a.k + 1) * a.r.n, (unsigned)a.r.n * 4u);
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      int rs = 0;
+      const bool isr = is_receiver(c0 + jj, rr, &rs);
+      ev[jj] = bld1(RD, isr ? (unsigned)rs * 4u : kDrop, 0u);
+    }
+  }
+  // strip slot of this step (FP32) or its FP32 staging (FP16 / BF16): the
+  // rows n2-nfd .. n2-1 (top) and nz-n2 .. nz-n2+nfd-1 (bottom), Hybrid3
+  // (kernel.cu:184-208), element 2a + (n2-1-z)*mnx + x-n2, resp.
+  // 2a + b + (z-(nz-n2))*mnx + x-n2, as strip_elem
+  Rsrc RS;
+  if (MODE == FR_FWD) {
+    const long long ss = a.st.ss;
+    const float* sb = a.st.fmt == 0
+        ? (const float*)a.st.base + ((long long)shot * a.st.nslot + a.k % a.st.nslot) * ss
+        : a.st.stage + ((long long)shot * 2 + (a.k & 1)) * ss;
+    RS = rsrc(sb, a.save_strips ? (unsigned)ss * 4u : 0u);
+  }
+  const int sa = g.mnz * g.nfd, sbb = g.nfd * sg.mnx;
+
+  const int t0 = z0 - 1;          // T row of the first iteration
+  const int niter = nrows + 2;    // T rows z0-1 .. z0+nrows
+  float4 q[U];
+#pragma unroll
+  for (int s = 0; s < NB + NFD + PD; ++s) q[s] = bld4(R1, vcol, rowm(t0 - NB + s));
+  uint4 kb[3];
+  float4 pmb[3], halb[3], u0b[6];
+#pragma unroll
+  for (int d = 0; d < PD; ++d) {
+    kb[d] = bld4u(RK, vcol, rowm(t0 + d));
+    pmb[d] = bld4(RP, vcol, rowm(t0 + d));
+    halb[d] = bld4(R1, vhal, rowm(t0 + d));
+    u0b[2 + d] = bld4s(R0, vcol, rowm(t0 + d));   // slot (it + 2) % 6 holds row t0 + it
+  }
+  u0b[0] = bld4s(R0, vcol, rowm(t0 - 2));
+  u0b[1] = bld4s(R0, vcol, rowm(t0 - 1));
+  CoefRow<NFD> cr[2][4];  // coefficient rows of T rows it, it+1 (slot it % 2)
+  cr[0][0].load(RC, kb[0].x);
+  cr[0][1].load(RC, kb[0].y);
+  cr[0][2].load(RC, kb[0].z);
+  cr[0][3].load(RC, kb[0].w);
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 T[3] = {z4, z4, z4};
+  float4 vpb[3] = {z4, z4, z4}, vqb[3] = {z4, z4, z4};  // slot (it + 1) % 3 holds row t0 + it
+  float wb[3] = {0.f, 0.f, 0.f};
+
+  for (int base = 0; base < niter; base += U) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int it = base + j;   // base is a multiple of U, so it % L == j % L for L | 12
+      if (it >= niter) break;
+      const int tt = t0 + it;
+      // (1) coefficient rows of T row tt+1 (their offsets arrived a row ago)
+      {
+        const uint4 kn = kb[(j + 1) % 3];
+        cr[(j + 1) % 2][0].load(RC, kn.x);
+        cr[(j + 1) % 2][1].load(RC, kn.y);
+        cr[(j + 1) % 2][2].load(RC, kn.z);
+        cr[(j + 1) % 2][3].load(RC, kn.w);
+      }
+      // (2) operands of later rows
+      q[(j + NB + NFD + PD) % U] = bld4(R1, vcol, rowm(tt + NFD + PD));
+      kb[(j + PD) % 3] = bld4u(RK, vcol, rowm(tt + PD));
+      pmb[(j + PD) % 3] = bld4(RP, vcol, rowm(tt + PD));
+      halb[(j + PD) % 3] = bld4(R1, vhal, rowm(tt + PD));
+      u0b[(j + 2 + PD) % 6] = bld4s(R0, vcol, rowm(tt + PD));
+      // one-way operands of row tt+1: velocity, the Q1 velocity (the cell
+      // itself on the top edge; Dv[(N2-x)*NX+y], the mirrored top-band cell,
+      // on the bottom edge, kernel.cu:132) and the blend weight of its ring
+      vpb[(j + 2) % 3] = bld4(RV, vcol, rowm(tt + 1));
+      if (BOT) vqb[(j + 2) % 3] = bld4(RV, vcol, rowc(NZ - 1 - (tt + 1)));
+      wb[(j + 2) % 3] = bld1(RW, (unsigned)min(max(N2 - ring(tt + 1), 0), N2) * 4u, 0u);
+      __builtin_amdgcn_sched_barrier(0);
+      // (3) horizontal neighbours of row tt through LDS
+      const float4 qc = q[(j + NB) % U];
+      lds[wv][hpos] = halb[j % 3];
+      lds[wv][H / 4 + lane] = qc;
+      wave_lds_sync();
+      float4 win[NWIN];
+#pragma unroll
+      for (int t = 0; t < NWIN; ++t) win[t] = lds[wv][lane + t];
+      wave_lds_sync();
+      // (4) pre-blend two-way values T of row tt (Add / BKAdd, kernel.cu:46-80, 339-380)
+      const float4 u0v = u0b[(j + 2) % 6], pv = pmb[j % 3];
+      float tn[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const CoefRow<NFD>& crc = cr[j % 2][jj];
+        const float u = compr(qc, jj);
+        float w1 = (float)(crc.c0d() * (double)u);
+#pragma unroll
+        for (int l = 1; l <= NFD; ++l) {
+          float s = compr(q[(j + NB - l + U) % U], jj) + compr(q[(j + NB + l) % U], jj);
+          s = __builtin_fmaf(s, m.hzx2_1, compr(win[(H + jj - l) >> 2], (H + jj - l) & 3));
+          s = s + compr(win[(H + jj + l) >> 2], (H + jj + l) & 3);
+          w1 = __builtin_fmaf(crc.c(l), s, w1);
+        }
+        float Tn;
+        if (MODE == FR_FWD) {
+          Tn = combine_p(u, compr(u0v, jj), compr(pv, jj), w1, a.f64comb != 0);
+          Tn = (tt == sg.r_u && c0 + jj == sg.r_x) ? Tn + a.wav : Tn;
+        } else {
+          Tn = combine_p(u, compr(u0v, jj), compr(pv, jj), w1, false);
+          Tn = (tt == a.r.s_z && ev[jj] != 0.0f) ? ev[jj] : Tn;
+        }
+        tn[jj] = Tn;
+      }
+      T[j % 3] = make_float4(tn[0], tn[1], tn[2], tn[3]);
+      // (5) one-way value and blend of row z = tt - 1 (written from the third row on)
+      const int z = tt - 1;
+      const float4 T0v = T[(j + 2) % 3];
+      const float4 Tinv = BOT ? T[(j + 1) % 3] : T[j % 3];
+      const float4 U1zv = q[(j + NB - 1) % U];
+      const float4 U1iv = BOT ? q[(j + NB - 2) % U] : q[(j + NB) % U];
+      const float4 U0zv = u0b[(j + 1) % 6];
+      const float4 U0iv = BOT ? u0b[j % 6] : u0b[(j + 2) % 6];
+      // lateral neighbours across lanes: x-1 of component 0, x+1 of component 3
+      const float Tl = __shfl_up(Tinv.w, 1), Tr = __shfl_down(Tinv.x, 1);
+      const float Ul = __shfl_up(U0zv.w, 1), Ur = __shfl_down(U0zv.x, 1);
+      const int ab = j % 3;
+      const float4 vpv = vpb[ab];
+      const float4 vqv = BOT ? vqb[ab] : vpb[ab];
+      const float wgt = wb[ab];
+      float val[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const float T0 = compr(T0v, jj), Tin = compr(Tinv, jj);
+        const float U1z = compr(U1zv, jj), U1i = compr(U1iv, jj);
+        const float U0z = compr(U0zv, jj), U0i = compr(U0iv, jj);
+        const float Til = jj == 0 ? Tl : compr(Tinv, jj - 1);
+        const float Tir = jj == 3 ? Tr : compr(Tinv, jj + 1);
+        const float U0l = jj == 0 ? Ul : compr(U0zv, jj - 1);
+        const float U0r = jj == 3 ? Ur : compr(U0zv, jj + 1);
+        const float vp = compr(vpv, jj), vq = compr(vqv, jj);
+        const float tv = m.taoh * vp;
+        const float t1 = tv * ((Tin - U0i) + U0z);
+        const float t2 = ((((-2.0f * U1z) + U0z) + Tin) - 2.0f * U1i) + U0i;
+        const float lap = ((((Tir - 2.0f * Tin) + Til) + U0r) - 2.0f * U0z) + U0l;
+        const float t3 = ((m.taoh2 * vq) * vq) * lap;
+        const float O = (1.0f / (tv + 1.0f)) * ((t1 - t2) + t3);
+        val[jj] = (1.0f - wgt) * T0 + wgt * O;
+      }
+      const bool rowout = it >= 2 && z < z0 + nrows;
+      const unsigned rz = rowm(z);
+      bst4s(R2, (rowout && full) ? vst : kDrop, rz, val[0], val[1], val[2], val[3]);
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+        bst1s_(R2, (rowout && !full && wr[jj]) ? vst + 4u * jj : kDrop, rz, val[jj]);
+      if (MODE == FR_FWD) {  // Hybrid3 (kernel.cu:184-208)
+        const bool srow = BOT ? (z >= NZ - N2 && z < NZ - N2 + g.nfd) : (z >= N2 - g.nfd && z < N2);
+        const int sbase = BOT ? 2 * sa + sbb + (z - (NZ - N2)) * sg.mnx - N2
+                              : 2 * sa + (N2 - 1 - z) * sg.mnx - N2;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+          bst1s(RS, (rowout && srow && wr[jj]) ? (unsigned)(sbase + c0 + jj) * 4u : kDrop, 0u, val[jj]);
+      }
+    }
+  }
+}
+
 template <int NFD, int EPI>
 __global__ __launch_bounds__(256) void k_interior(const IntArgs a) {
   interior_body<NFD, EPI>(a, blockIdx.x);
+}
+
+// One wave per float4 top/bottom record (frame_tb4_body).
+template <int NFD, int MODE>
+__global__ __launch_bounds__(256) void k_frame4(const FrameArgs a) {
+  const int ri = (int)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (ri >= a.nrec) return;
+  const int4 rec = a.recs[ri];
+  if ((rec.z & 15) == 0) frame_tb4_body<NFD, MODE, false>(a, rec);
+  else frame_tb4_body<NFD, MODE, true>(a, rec);
 }
 
 template <int NFD, int MODE>
@@ -1434,29 +1679,79 @@ int frame_segment_rows(int nfd) {
   const int nb = nfd > 2 ? nfd : 2;
   return 2 * (nb + nfd + 1 + 2) - 2;  // two register-queue cycles per side run
 }
-void frame_records(const Grid& g, int nx, int shot, int seg_tb, int seg, int4* out, int* count) {
+// Float4 windows of the top and bottom bands (k_frame4): first column n2,
+// 252 apart, writing [cs+1, min(cs+253, nx-n2-1)).  None when the band's edge
+// cells do not reach one window or n2 leaves no room for the column halo.
+static int tb4_windows(const Grid& g, int nx) {
+  const int wend = nx - g.n2 - 1;
+  if (g.n2 < 12 || wend <= g.n2 + 1) return 0;
+  return (wend - (g.n2 + 1) + 251) / 252;
+}
+
+void frame_records(const Grid& g, int nx, int shot, int seg_tb, int seg, int kind, int4* out,
+                   int* count) {
   int n = 0;
-  auto put = [&](int z0, int cx0, int band, int rows) {
-    if (out) out[n] = make_int4(z0, cx0, band | (rows << 4), shot);
+  auto put = [&](int z0, int cx0, int band, int rows, int ncol) {
+    if (out) out[n] = make_int4(z0, cx0, band | (rows << 4) | (ncol << 20), shot);
     ++n;
   };
-  for (int z0 = 0; z0 < g.n2; z0 += seg_tb)  // top band in runs of seg_tb rows
-    for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, 0, std::min(seg_tb, g.n2 - z0));
-  for (int z0 = g.nz - g.n2; z0 < g.nz; z0 += seg_tb)
-    for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, 1, std::min(seg_tb, g.nz - z0));
+  const int nw = kind == FRAME_ALL ? 0 : tb4_windows(g, nx);
+  // top band, bottom band, in runs of seg_tb rows (the float4 records: one
+  // run of the whole band, each wave writing 252 columns)
+  const int run = kind == FRAME_VEC ? g.n2 : seg_tb;
+  for (int b = 0; b < 2; ++b) {
+    const int zb = b == 0 ? 0 : g.nz - g.n2, ze = b == 0 ? g.n2 : g.nz;
+    for (int z0 = zb; z0 < ze; z0 += run) {
+      const int rows = std::min(run, ze - z0);
+      if (nw == 0) {
+        if (kind != FRAME_VEC)
+          for (int cx0 = -1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, b, rows, 0);
+      } else if (kind == FRAME_VEC) {
+        for (int w = 0; w < nw; ++w) put(z0, g.n2 + 252 * w, b, rows, 0);
+      } else {  // the columns left to the scalar kernel: [0, n2+1) and [nx-n2-1, nx)
+        for (int c = 0; c < g.n2 + 1; c += 62) put(z0, c - 1, b, rows, std::min(62, g.n2 + 1 - c));
+        for (int c = nx - g.n2 - 1; c < nx; c += 62) put(z0, c - 1, b, rows, std::min(62, nx - c));
+      }
+    }
+  }
+  if (kind == FRAME_VEC) {
+    *count = n;
+    return;
+  }
   // side bands: both in one wave per row run (band 4) when each fits in half
   // a wave with its inner neighbour and the halves do not overlap
   const bool pack = g.n2 <= 30 && nx >= 64;
   for (int z0 = g.n2; z0 < g.nz - g.n2; z0 += seg) {
     const int rows = std::min(seg, g.nz - g.n2 - z0);
     if (pack) {
-      put(z0, 0, 4, rows);
+      put(z0, 0, 4, rows, 0);
       continue;
     }
-    for (int cx0 = -1; cx0 + 1 < g.n2; cx0 += 62) put(z0, cx0, 2, rows);
-    for (int cx0 = nx - g.n2 - 1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, 3, rows);
+    for (int cx0 = -1; cx0 + 1 < g.n2; cx0 += 62) put(z0, cx0, 2, rows, 0);
+    for (int cx0 = nx - g.n2 - 1; cx0 + 1 < nx; cx0 += 62) put(z0, cx0, 3, rows, 0);
   }
   *count = n;
+}
+
+bool frame4_supported(int nfd) { return nfd >= 1 && nfd <= 6; }
+
+hipError_t launch_frame4(int mode, int nfd, FrameArgs a, int nshot, int nrec, hipStream_t s) {
+  a.nshot = nshot;
+  a.nrec = nrec;
+  if (nrec == 0) return hipSuccess;
+  dim3 grid((nrec + 3) / 4), block(256);  // one wave per record
+#define CASE(N)                                                                          \
+  case N:                                                                                \
+    if (mode == FR_FWD) hipLaunchKernelGGL((k_frame4<N, FR_FWD>), grid, block, 0, s, a); \
+    else if (mode == FR_RCV) hipLaunchKernelGGL((k_frame4<N, FR_RCV>), grid, block, 0, s, a); \
+    else return hipErrorInvalidValue;                                                    \
+    break;
+  switch (nfd) {
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6)
+    default: return hipErrorInvalidValue;
+  }
+#undef CASE
+  return hipGetLastError();
 }
 
 hipError_t launch_frame(int mode, int nfd, FrameArgs a, int nshot, int nrec, hipStream_t s) {
