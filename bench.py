@@ -848,6 +848,10 @@ def main():
     fusedk = {} if not a.fuse_frame else {
         hip.K_FWD_INTERIOR: (hip.K_FWD_FRAME, "k_step<EPI_FWD+FR_FWD>"),
         hip.K_RCVI_INTERIOR: (hip.K_RCV_FRAME, "k_step<EPI_RCVI+FR_RCV>")}
+    # Without a checkpoint snapshot (none here) and with separate launches, the
+    # receiver frame launch of every backward step also restores the strips of
+    # the reconstructed field (k_strip_restore's work and bytes).
+    restore_in_frame = not a.fuse_frame
     for k, name in hip.KERNEL_NAMES.items():
         try:
             ms, n = ctx.kernel_time(k)
@@ -858,6 +862,9 @@ def main():
             if k in fusedk and ctx.kernel_time(fusedk[k][0])[1] == 0:
                 b += algorithmic_bytes(fusedk[k][0], d, cfg.n2, cfg.nfdmax, a.shots, cfg.n)
                 name = fusedk[k][1]
+            if k == hip.K_RCV_FRAME and restore_in_frame:
+                b += algorithmic_bytes(hip.K_STRIP_RESTORE, d, cfg.n2, cfg.nfdmax, a.shots, cfg.n)
+                name = "k_frame<FR_RCV>+k_strip_restore"
             kinds[name] = dict(total_ms=ms, launches=n, avg_ms=ms / n,
                                gbs=b / (ms / n * 1e-3) / 1e9, bytes_per_launch=b)
     dom = max(kinds, key=lambda k: kinds[k]["total_ms"])

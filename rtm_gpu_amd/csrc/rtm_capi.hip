@@ -249,6 +249,8 @@ void fill_frame_args(rtm_ctx* c, FrameArgs& a) {
   a.data = c->d_data;
   a.st = c->st;
   a.save_strips = 0;
+  a.rf = nullptr;
+  a.rbps = 0;
   a.ua = a.ub = a.uc = nullptr;
   a.adjt = 0;
 }
@@ -366,7 +368,12 @@ int backward_step(rtm_ctx* c, int k) {
     a.zs = zs_for(c, esrc);
     HIPCHK(c, launch_interior(esrc, c->nfd, a, c->nshot, c->stream));
   }
-  {  // strips of slot k into the rings of u(k): BKEqual of step k-1 (kernel.cu:892)
+  // strips of slot k into the rings of u(k): BKEqual of step k-1 (kernel.cu:892).
+  // Nothing in the receiver step reads those rings, so on ordinary steps the
+  // receiver frame launch carries the restore; a checkpoint step restores
+  // before its snapshot replaces u(k).
+  const bool restore_in_frame = c->snap_slot[k] < 0 && !fused(c, ercv, FR_RCV);
+  if (!restore_in_frame) {
     Timed t(c, RTM_K_STRIP_RESTORE);
     HIPCHK(c, launch_strip_restore(c->g, c->F[c->f2], c->st, c->d_geo, k, c->nshot, c->stream));
   }
@@ -385,6 +392,10 @@ int backward_step(rtm_ctx* c, int k) {
     fill_frame_args(c, f);
     f.u1 = c->Q[c->q1]; f.u0 = c->Q[c->q0]; f.u2 = c->Q[c->q2];
     f.wav = 0.f; f.k = k;
+    if (restore_in_frame) {  // u(k) is F[f1] after the rotation
+      f.rf = c->F[c->f1];
+      f.rbps = strip_restore_blocks(c->g);
+    }
     const int kind = split ? RTM_K_RCVI_INTERIOR : RTM_K_RCV_INTERIOR;
     if (fused(c, ercv, FR_RCV)) {
       Timed t(c, kind);  // the fused launch counts as the interior kind

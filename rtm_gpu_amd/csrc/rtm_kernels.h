@@ -129,6 +129,10 @@ struct FrameArgs {
   int save_strips;
   const float *ua, *ub, *uc;  // Born background fields (as in IntArgs)
   int adjt;            // adjoint with per-cell rows: interior cells take the transposed stencil
+  // receiver frame only: rbps > 0 appends rbps blocks per shot that restore
+  // the strips of slot k into the rings of field rf (k_strip_restore's work)
+  float* rf;
+  int rbps;
 };
 
 // Launchers (rtm_kernels.hip).  nfd_inst is the compile-time stencil
@@ -181,6 +185,8 @@ hipError_t launch_strip_save(const Grid& g, const float* f, Strips st, const Sho
                              int nshot, hipStream_t s);
 hipError_t launch_strip_restore(const Grid& g, float* f, Strips st, const ShotGeo* geo, int k,
                                 int nshot, hipStream_t s);
+// Restore blocks per shot when the receiver frame launch carries the restore.
+int strip_restore_blocks(const Grid& g);
 hipError_t launch_strip_finish(const Grid& g, Strips st, const ShotGeo* geo, int k, int nshot,
                                hipStream_t s);
 hipError_t launch_snap_save(const Grid& g, const float* f, Snaps sn, const ShotGeo* geo, int slot,

@@ -177,29 +177,26 @@ __device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest e
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float(((unsigned)b) << 16); }
 
-// Strip element index -> grid cell of a shot of width nx (interior mnx).
-// Slot layout: lf[mnz][nfd], rt[mnz][nfd], up[nfd][mnx], dw[nfd][mnx]
+// Strip slot layout: lf[mnz][nfd], rt[mnz][nfd], up[nfd][mnx], dw[nfd][mnx]
 // (Dlf/Drt/Dup/Ddw of kernel.cu:199-205); a slot holds ss_of(mnx) elements.
 __device__ __forceinline__ long long strip_size(const Grid& g, int mnx) {
   return 2LL * g.nfd * (g.mnz + mnx);
 }
-__device__ __forceinline__ void strip_cell(const Grid& g, int nx, int mnx, long long e, int* z, int* x) {
-  const long long a = (long long)g.mnz * g.nfd, b = (long long)g.nfd * mnx;
-  if (e < a) {
-    int r = (int)(e / g.nfd), d = (int)(e % g.nfd);
-    *z = g.n2 + r; *x = g.n2 - 1 - d;
-  } else if (e < 2 * a) {
-    e -= a;
-    int r = (int)(e / g.nfd), d = (int)(e % g.nfd);
-    *z = g.n2 + r; *x = nx - g.n2 + d;
-  } else if (e < 2 * a + b) {
-    e -= 2 * a;
-    int d = (int)(e / mnx), c = (int)(e % mnx);
-    *z = g.n2 - 1 - d; *x = g.n2 + c;
+// Strip element index -> grid cell of a shot of width nx (interior mnx), in
+// 32-bit arithmetic (a slot holds fewer than 2^31 elements).
+__device__ __forceinline__ void strip_cell(const Grid& g, int nx, int mnx, unsigned e, int* z,
+                                           int* x) {
+  const unsigned nfd = (unsigned)g.nfd, a = (unsigned)g.mnz * nfd, b = nfd * (unsigned)mnx;
+  if (e < 2 * a) {
+    const unsigned side = e >= a ? 1u : 0u, ee = e - side * a;
+    const unsigned r = ee / nfd, d = ee - r * nfd;
+    *z = g.n2 + (int)r;
+    *x = side ? nx - g.n2 + (int)d : g.n2 - 1 - (int)d;
   } else {
-    e -= 2 * a + b;
-    int d = (int)(e / mnx), c = (int)(e % mnx);
-    *z = g.nz - g.n2 + d; *x = g.n2 + c;
+    const unsigned bot = e >= 2 * a + b ? 1u : 0u, ee = e - 2 * a - bot * b;
+    const unsigned d = ee / (unsigned)mnx, c = ee - d * (unsigned)mnx;
+    *z = bot ? g.nz - g.n2 + (int)d : g.n2 - 1 - (int)d;
+    *x = g.n2 + (int)c;
   }
 }
 // Grid cell -> strip element index, or -1.
@@ -233,6 +230,22 @@ __device__ __forceinline__ float strip_get(const Strips& st, int shot, int k, lo
   float sc = st.scale[slot];
   if (st.fmt == 1) return __half2float(__ushort_as_half(h)) * sc;
   return bf2f(h) * sc;
+}
+// Strips of slot k into the rings of field f: block b of a launch with bps
+// 256-thread blocks per shot restores part of shot b / bps.
+__device__ __forceinline__ void strip_restore_body(const Grid& g, float* f, const Strips& st,
+                                                   const ShotGeo* geo, int k, unsigned b,
+                                                   unsigned bps) {
+  const int shot = (int)(b / bps);
+  const unsigned blk = b - (unsigned)shot * bps;
+  const ShotGeo sg = geo[shot];
+  const unsigned ss = (unsigned)strip_size(g, sg.mnx);
+  float* F = f + (long long)shot * g.fstride + g.x0;
+  for (unsigned e = blk * 256u + threadIdx.x; e < ss; e += bps * 256u) {
+    int z, x;
+    strip_cell(g, sg.nx, sg.mnx, e, &z, &x);
+    F[(long)z * g.pitch + x] = strip_get(st, shot, k, (long long)e);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1209,8 +1222,18 @@ __global__ __launch_bounds__(256) void k_frame4(const FrameArgs a) {
   else frame_tb4_body<NFD, MODE, true>(a, rec);
 }
 
+// Receiver frame launches may carry the strip restore of the reconstructed
+// field (FrameArgs::rf): its blocks follow the record blocks and fill the wave
+// slots the frame's few long waves leave free.
 template <int NFD, int MODE>
 __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
+  if (MODE == FR_RCV && a.rbps > 0) {
+    const unsigned nfb = ((unsigned)a.nrec + 3u) >> 2;
+    if (blockIdx.x >= nfb) {
+      strip_restore_body(a.g, a.rf, a.st, a.geo, a.k, blockIdx.x - nfb, (unsigned)a.rbps);
+      return;
+    }
+  }
   frame_body<NFD, MODE>(a, blockIdx.x);
 }
 
@@ -1282,13 +1305,12 @@ __global__ void k_fwd_begin(Grid g, float* f0, float* f1, const ShotGeo* geo, fl
 __global__ void k_strip_save(Grid g, const float* f, Strips st, const ShotGeo* geo, int k) {
   const int shot = blockIdx.y;
   const ShotGeo sg = geo[shot];
-  const long long ss = strip_size(g, sg.mnx);
+  const unsigned ss = (unsigned)strip_size(g, sg.mnx);
   const float* F = f + (long long)shot * g.fstride + g.x0;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ss;
-       e += (long long)gridDim.x * blockDim.x) {
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < ss; e += gridDim.x * blockDim.x) {
     int z, x;
     strip_cell(g, sg.nx, sg.mnx, e, &z, &x);
-    strip_put(st, shot, k, e, F[(long)z * g.pitch + x]);
+    strip_put(st, shot, k, (long long)e, F[(long)z * g.pitch + x]);
   }
 }
 
@@ -1328,17 +1350,9 @@ __global__ void k_strip_finish(Grid g, Strips st, const ShotGeo* geo, int k) {
   }
 }
 
-__global__ void k_strip_restore(Grid g, float* f, Strips st, const ShotGeo* geo, int k) {
-  const int shot = blockIdx.y;
-  const ShotGeo sg = geo[shot];
-  const long long ss = strip_size(g, sg.mnx);
-  float* F = f + (long long)shot * g.fstride + g.x0;
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < ss;
-       e += (long long)gridDim.x * blockDim.x) {
-    int z, x;
-    strip_cell(g, sg.nx, sg.mnx, e, &z, &x);
-    F[(long)z * g.pitch + x] = strip_get(st, shot, k, e);
-  }
+__global__ __launch_bounds__(256) void k_strip_restore(Grid g, float* f, Strips st,
+                                                       const ShotGeo* geo, int k) {
+  strip_restore_body(g, f, st, geo, k, blockIdx.y * gridDim.x + blockIdx.x, gridDim.x);
 }
 
 // ---- checkpoint snapshots -------------------------------------------------
@@ -1757,7 +1771,9 @@ hipError_t launch_frame4(int mode, int nfd, FrameArgs a, int nshot, int nrec, hi
 hipError_t launch_frame(int mode, int nfd, FrameArgs a, int nshot, int nrec, hipStream_t s) {
   a.nshot = nshot;
   a.nrec = nrec;
-  dim3 grid((nrec + 3) / 4), block(256);  // one wave per (region run, shot) record
+  if (mode != FR_RCV) a.rbps = 0;
+  // one wave per (region run, shot) record, then the restore blocks
+  dim3 grid((nrec + 3) / 4 + (a.rbps > 0 ? a.rbps * nshot : 0)), block(256);
 #define CASE(N)                                                                          \
   case N:                                                                                \
     if (mode == FR_FWD) hipLaunchKernelGGL((k_frame<N, FR_FWD>), grid, block, 0, s, a);  \
@@ -1864,6 +1880,7 @@ hipError_t launch_fwd_begin(const Grid& g, float* f0, float* f1, const ShotGeo* 
   return hipGetLastError();
 }
 static inline long long strip_size_host(const Grid& g) { return 2LL * g.nfd * (g.mnz + g.mnx); }
+int strip_restore_blocks(const Grid& g) { return (int)blocks_for(strip_size_host(g), 1024); }
 hipError_t launch_strip_save(const Grid& g, const float* f, Strips st, const ShotGeo* geo, int k,
                              int nshot, hipStream_t s) {
   hipLaunchKernelGGL(k_strip_save, dim3(blocks_for(strip_size_host(g)), nshot), dim3(256), 0, s, g,
