@@ -726,6 +726,10 @@ def main():
     ap.add_argument("--fuse-frame", action="store_true",
                     help="run each forward and receiver step's interior and frame kernels as "
                          "one launch (rtm_params.fuse_frame)")
+    ap.add_argument("--no-split", action="store_true",
+                    help="keep the whole frame in the one-column kernel (rtm_params.frame_split "
+                         "= 0; by default the split starts on and the autotuner keeps it only "
+                         "where it is faster)")
     ap.add_argument("--host-gathers", action="store_true",
                     help="every timed step uploads the gathers from host memory (the PCIe-"
                          "inclusive rate of a caller that hands over host buffers)")
@@ -783,7 +787,7 @@ def main():
                       ilste=0, icompen=1, s_l=d.s_l, s_r=d.s_r, s_z=d.s_z, ds=1, n=cfg.n,
                       tao2=d.tao2, h2=d.h2, taoh=d.taoh, taoh2=d.taoh2, hzx2_1=d.hzx2_1,
                       max_shots=a.shots, strip_format=strip_fmt, record=1, zseg=a.zseg,
-                      fuse_frame=1 if a.fuse_frame else 0)
+                      fuse_frame=1 if a.fuse_frame else 0, frame_split=0 if a.no_split else 1)
     log(f"[bench] context: {ctx.device_bytes() / 2**30:.1f} GiB on device")
     mt, mm = P["models"]["true"], P["models"]["mig"]
     ctx.set_wavelet(P["wav"])
@@ -892,7 +896,9 @@ def main():
                    "strips": a.strips, "coefficients": "adaptive LS",
                    "velocity_bins": mm["nbins"], "rows_per_wave": zseg,
                    "frame_runs": list(ctx.frame_runs()),
-                   "frame_launch": "one launch with the interior" if a.fuse_frame else "separate"},
+                   "frame_launch": "one launch with the interior" if a.fuse_frame else "separate",
+                   "frame_split": ("off" if a.no_split else
+                                   "on" if a.zsegs or a.zseg > 0 else "autotuned")},
         "shots_per_s": round(a.shots * a.steps / el, 3),
         "hbm_frac_whole_job": round(total_bytes / el / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(kd["gbs"], 1),

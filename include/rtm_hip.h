@@ -51,7 +51,10 @@ enum {
    * RTM_FIELD_Q holds mu = P*lambda and RTM_FIELD_I1 the accumulator G. */
 };
 
-/* Kernel kinds for rtm_kernel_time(). */
+/* Kernel kinds for rtm_kernel_time().  A frame kind times the frame launches
+ * of one step together; on migration steps without a checkpoint snapshot
+ * (and with separate launches) RTM_K_RCV_FRAME includes the strip restore of
+ * the reconstructed field, which RTM_K_STRIP_RESTORE then does not count. */
 enum {
   RTM_K_FWD_INTERIOR = 0, RTM_K_FWD_FRAME = 1, RTM_K_SRC_INTERIOR = 2, RTM_K_STRIP_RESTORE = 3,
   RTM_K_RCV_INTERIOR = 4, RTM_K_RCV_FRAME = 5, RTM_K_BORN_INTERIOR = 6, RTM_K_BORN_FRAME = 7,
@@ -94,6 +97,14 @@ typedef struct rtm_params {
                             are the same bit for bit; on the reverse-VSP batch the
                             fused receiver step is 2% faster and the fused forward
                             step 11% slower, the whole job within 2% either way. */
+  int frame_split;       /* not in the reference: 1 runs the edge cells of the top and
+                            bottom bands of the forward and receiver frames in a
+                            four-column-per-lane kernel beside the one-column kernel
+                            (n2 >= 12, stencil half-lengths up to 6); 0 (default) runs
+                            the whole frame in the one-column kernel.  Results are the
+                            same bit for bit; the split pays only on large batches (its
+                            launch takes about as long for one shot as for 128), so
+                            rtm_autotune times both and keeps the faster. */
 } rtm_params;
 
 /* One shot of a ragged batch (uneven apertures; not in the reference, whose
@@ -273,7 +284,8 @@ int rtm_data_dot_noalloc(const float* x, const float* y, long long n, long long 
  * rtm_kernel_time counters.  Rows per wave trade the vertical halo re-read
  * (2*NFD rows per segment) against the number of waves in flight.  Then the
  * frame kernels' row runs (rtm_set_frame_runs) are timed the same way, top and
- * bottom bands first, on the sum of both frame kernels' times. */
+ * bottom bands first, on the sum of both frame kernels' times, and last the
+ * frame split (rtm_params.frame_split) on and off. */
 int rtm_autotune(rtm_ctx* ctx, int nsteps, int* zseg);
 /* The autotuner's three choices set by hand: rows per wave of the forward,
  * the source-reconstruction and the receiver-step kernels (Born and the

@@ -292,10 +292,11 @@ static bool fused(const rtm_ctx* c, int epi, int mode) {
 
 // The frame of a forward or receiver step: split between k_frame (the band
 // ends at the corners, the side bands) and k_frame4 (the edge cells of the top
-// and bottom bands) when the batch has float4 records, else k_frame on the
-// FRAME_ALL records.  The two launches write disjoint cells.
+// and bottom bands) when rtm_params.frame_split asks for it and the batch has
+// float4 records, else k_frame on the FRAME_ALL records.  The two launches
+// write disjoint cells.
 hipError_t launch_frame_split(rtm_ctx* c, int mode, FrameArgs f) {
-  if (c->nvec > 0 && frame4_supported(c->nfd)) {
+  if (c->p.frame_split && c->nvec > 0 && frame4_supported(c->nfd)) {
     FrameArgs fs = f;
     fs.recs = c->d_tiles + c->ntiles;
     const hipError_t e = launch_frame(mode, c->nfd, fs, c->nshot, c->nsplit, c->stream);
@@ -1573,6 +1574,20 @@ int rtm_autotune(rtm_ctx* c, int nsteps, int* zseg) {
       run = cand[which][bi];
     }
     if (rc == RTM_OK) rc = upload_frame_records(c);
+    // the frame split on and off (the four-column launch pays only when the
+    // batch fills the GPU with frame waves)
+    if (rc == RTM_OK && c->nvec > 0 && frame4_supported(c->nfd) && !fused(c, fwd, FR_FWD)) {
+      double tot[2] = {0.0, 0.0};
+      for (int round = 0; round < kRounds && rc == RTM_OK; ++round)
+        for (int i = 0; i < 2 && rc == RTM_OK; ++i) {
+          const int on = (round & 1) ? 1 - i : i;
+          c->p.frame_split = on;
+          double ms = 0.0;
+          rc = measure(fmask, &ms);
+          tot[on] += ms;
+        }
+      c->p.frame_split = tot[1] < tot[0] ? 1 : 0;
+    }
   }
   c->timing = saved_timing;
   c->check_every = saved_check;

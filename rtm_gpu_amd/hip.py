@@ -33,7 +33,7 @@ class RtmParams(C.Structure):
                [(n, C.c_float) for n in ("tao2", "h2", "taoh", "taoh2", "hzx2_1")] + \
                [(n, C.c_int) for n in ("max_shots", "strip_format", "record", "zseg",
                                        "snap_every", "stack_nx", "check_every",
-                                       "fuse_frame")]
+                                       "fuse_frame", "frame_split")]
 
 
 class RtmShotGeom(C.Structure):

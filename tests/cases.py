@@ -82,14 +82,14 @@ def oracle_engine(case, shot=0, prec=32, record=0, nthreads=1, snap_every=0):
     return ob.Engine(cfg, case["model"], prec=prec)
 
 
-def hip_context(case, nshot=None, record=0, strip_format=0, zseg=0, snap_every=0):
+def hip_context(case, nshot=None, record=0, strip_format=0, zseg=0, snap_every=0, frame_split=0):
     from rtm_gpu_amd import hip
     cfg = case["cfg"]
     nshot = nshot or len(case["r_u"])
     keys = ("nz", "nx", "n2", "nfdmax", "nt", "nt2", "ilste", "icompen", "s_l", "s_r", "s_z",
             "ds", "n", "tao2", "h2", "taoh", "taoh2", "hzx2_1")
     ctx = hip.Context(**{k: cfg[k] for k in keys}, max_shots=nshot, strip_format=strip_format,
-                      record=record, zseg=zseg, snap_every=snap_every)
+                      record=record, zseg=zseg, snap_every=snap_every, frame_split=frame_split)
     m = case["model"]
     ctx.set_model(m["v"], m["r1"], m["w"], m["c"], m["index"], m["nvel"], m["vmin"], m["dv"])
     ctx.set_wavelet(m["wav"])

@@ -182,7 +182,8 @@ def test_many_identical_shots_agree():
     cfg = case["cfg"]
     keys = ("nz", "nx", "n2", "nfdmax", "nt", "nt2", "ilste", "icompen", "s_l", "s_r", "s_z",
             "ds", "n", "tao2", "h2", "taoh", "taoh2", "hzx2_1")
-    ctx = hip.Context(**{k: cfg[k] for k in keys}, max_shots=ns)
+    # the frame split on: the band edges in the four-column kernel (n2 = 12)
+    ctx = hip.Context(**{k: cfg[k] for k in keys}, max_shots=ns, frame_split=1)
     ctx.set_model(m["v"], m["r1"], m["w"], m["c"], m["index"], m["nvel"], m["vmin"], m["dv"])
     ctx.set_wavelet(m["wav"])
     ctx.set_shots(case["r_u"] * ns, case["r_x"] * ns)
@@ -290,6 +291,37 @@ def test_frame_runs_keep_results():
         assert all(np.array_equal(a, b) for a, b in zip(o[2], out[0][2]))
     with pytest.raises(hip.RtmError):
         hip_context(case, nshot=1).set_frame_runs(n2 + 1, 20)
+
+
+def test_frame_split_keeps_results():
+    """The band edges of the forward and receiver frames in the four-column
+    kernel (frame_split = 1; n2 = 12, several float4 records per band run)
+    give the fields, strips and images of the one-column kernel bit for bit,
+    with whole-band and shortened frame runs (the receiver frame launch also
+    restores the strips of the reconstructed field)."""
+    from rtm_gpu_amd import hip
+    case = build_case(name="split", mod_nz=60, mod_nx=560, n2=12, nfdmax=4, nt=200, nshot=3,
+                      depth_step=20.0, seed=7)
+    seis = np.stack([synth_gather(case, s) for s in range(3)])
+    cfg = case["cfg"]
+    for runs in (None, ((cfg["n2"] + 2) // 3, 10)):
+        out = []
+        for split in (0, 1):
+            ctx = hip_context(case, nshot=3, frame_split=split)
+            if runs:
+                ctx.set_frame_runs(*runs)
+            ctx.set_gathers(seis)
+            ctx.forward_begin()
+            ctx.forward_advance(cfg["nt"] // 2)
+            fwd = [ctx.field(hip.FIELD_U, shot=s) for s in range(3)]
+            strip = ctx.strip(2, cfg["nt"] // 2 - 1)
+            ctx.migrate()
+            out.append((fwd, strip, [ctx.field(w, shot=s) for s in range(3)
+                                     for w in (hip.FIELD_I1, hip.FIELD_I2, hip.FIELD_Q)]))
+            ctx.close()
+        assert all(np.array_equal(a, b) for a, b in zip(out[1][0], out[0][0]))
+        assert np.array_equal(out[1][1], out[0][1])
+        assert all(np.array_equal(a, b) for a, b in zip(out[1][2], out[0][2]))
 
 
 def test_fused_step_launch_keeps_results():
