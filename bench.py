@@ -450,6 +450,9 @@ def run_lsrtm(a):
         ms, n = ctx.kernel_time(k)
         if n:
             b = algorithmic_bytes(k, d, cfg.n2, cfg.nfdmax, a.shots, cfg.n)
+            if k == hip.K_ADJ_FRAME:  # the adjoint frame launch carries the strip restore
+                b += algorithmic_bytes(hip.K_STRIP_RESTORE, d, cfg.n2, cfg.nfdmax, a.shots, cfg.n)
+                name = "k_frame<FR_ADJ>+k_strip_restore"
             kinds[name] = dict(avg_ms=round(ms / n, 5), launches=n,
                                GBps=round(b / (ms / n * 1e-3) / 1e9, 1) if b else None)
     ad = kinds.get("k_interior<EPI_ADJ>", {})

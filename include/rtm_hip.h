@@ -52,9 +52,10 @@ enum {
 };
 
 /* Kernel kinds for rtm_kernel_time().  A frame kind times the frame launches
- * of one step together; on migration steps without a checkpoint snapshot
- * (and with separate launches) RTM_K_RCV_FRAME includes the strip restore of
- * the reconstructed field, which RTM_K_STRIP_RESTORE then does not count. */
+ * of one step together; on migration and adjoint steps without a checkpoint
+ * snapshot (and with separate launches) RTM_K_RCV_FRAME / RTM_K_ADJ_FRAME
+ * include the strip restore of the reconstructed field, which
+ * RTM_K_STRIP_RESTORE then does not count. */
 enum {
   RTM_K_FWD_INTERIOR = 0, RTM_K_FWD_FRAME = 1, RTM_K_SRC_INTERIOR = 2, RTM_K_STRIP_RESTORE = 3,
   RTM_K_RCV_INTERIOR = 4, RTM_K_RCV_FRAME = 5, RTM_K_BORN_INTERIOR = 6, RTM_K_BORN_FRAME = 7,

@@ -456,7 +456,9 @@ int born_step(rtm_ctx* c, int k) {
 // One update of the adjoint field: mu(k) from mu(k+1), mu(k+2) with P*r(k)
 // at the receivers; with image, G += D2u * mu(k+2), D2u = (u(k+2) - 2u(k+1))
 // + u(k) (the background rotation leaves u(k), u(k+1), u(k+2) in f1, f0, f2).
-int adjoint_update(rtm_ctx* c, int k, bool image) {
+// restore: when not null, the frame launch also restores the strips of slot k
+// into the rings of that field (the background u(k) of adjoint_step).
+int adjoint_update(rtm_ctx* c, int k, bool image, float* restore = nullptr) {
   {
     IntArgs a;
     fill_int_args(c, a);
@@ -477,6 +479,10 @@ int adjoint_update(rtm_ctx* c, int k, bool image) {
     a.u1 = c->Q[c->q1]; a.u0 = c->Q[c->q0]; a.u2 = c->Q[c->q2];
     a.wav = 0.f; a.k = k;
     a.adjt = c->p.ilste == 0;
+    if (restore) {
+      a.rf = restore;
+      a.rbps = strip_restore_blocks(c->g);
+    }
     Timed t(c, RTM_K_ADJ_FRAME);
     HIPCHK(c, launch_frame(FR_ADJ, c->nfd, a, c->nshot, c->ntiles, c->stream));
   }
@@ -501,14 +507,17 @@ int adjoint_step(rtm_ctx* c, int k) {
     a.zs = zs_for(c, EPI_SRC);
     HIPCHK(c, launch_interior(EPI_SRC, c->nfd, a, c->nshot, c->stream));
   }
-  {
+  // the strip restore of u(k): in the adjoint frame launch (nothing in the
+  // adjoint update reads the rings of u(k)), except before a snapshot
+  const bool restore_in_frame = c->snap_slot[k] < 0;
+  if (!restore_in_frame) {
     Timed t(c, RTM_K_STRIP_RESTORE);
     HIPCHK(c, launch_strip_restore(c->g, c->F[c->f2], c->st, c->d_geo, k, c->nshot, c->stream));
   }
   if (c->snap_slot[k] >= 0)
     HIPCHK(c, launch_snap_restore(c->g, c->F[c->f2], c->sn, c->d_geo, c->snap_slot[k], c->nshot, c->stream));
   { int t = c->f0; c->f0 = c->f1; c->f1 = c->f2; c->f2 = t; }
-  return adjoint_update(c, k, true);
+  return adjoint_update(c, k, true, restore_in_frame ? c->F[c->f1] : nullptr);
 }
 
 // Frame records of a grid nx columns wide, tagged with the shot: top and

@@ -129,8 +129,9 @@ struct FrameArgs {
   int save_strips;
   const float *ua, *ub, *uc;  // Born background fields (as in IntArgs)
   int adjt;            // adjoint with per-cell rows: interior cells take the transposed stencil
-  // receiver frame only: rbps > 0 appends rbps blocks per shot that restore
-  // the strips of slot k into the rings of field rf (k_strip_restore's work)
+  // receiver and adjoint frames only: rbps > 0 appends rbps blocks per shot
+  // that restore the strips of slot k into the rings of field rf
+  // (k_strip_restore's work)
   float* rf;
   int rbps;
 };

@@ -1222,12 +1222,12 @@ __global__ __launch_bounds__(256) void k_frame4(const FrameArgs a) {
   else frame_tb4_body<NFD, MODE, true>(a, rec);
 }
 
-// Receiver frame launches may carry the strip restore of the reconstructed
-// field (FrameArgs::rf): its blocks follow the record blocks and fill the wave
-// slots the frame's few long waves leave free.
+// Receiver and adjoint frame launches may carry the strip restore of the
+// reconstructed field (FrameArgs::rf): its blocks follow the record blocks and
+// fill the wave slots the frame's few long waves leave free.
 template <int NFD, int MODE>
 __global__ __launch_bounds__(256) void k_frame(const FrameArgs a) {
-  if (MODE == FR_RCV && a.rbps > 0) {
+  if ((MODE == FR_RCV || MODE == FR_ADJ) && a.rbps > 0) {
     const unsigned nfb = ((unsigned)a.nrec + 3u) >> 2;
     if (blockIdx.x >= nfb) {
       strip_restore_body(a.g, a.rf, a.st, a.geo, a.k, blockIdx.x - nfb, (unsigned)a.rbps);
@@ -1771,7 +1771,7 @@ hipError_t launch_frame4(int mode, int nfd, FrameArgs a, int nshot, int nrec, hi
 hipError_t launch_frame(int mode, int nfd, FrameArgs a, int nshot, int nrec, hipStream_t s) {
   a.nshot = nshot;
   a.nrec = nrec;
-  if (mode != FR_RCV) a.rbps = 0;
+  if (mode != FR_RCV && mode != FR_ADJ) a.rbps = 0;
   // one wave per (region run, shot) record, then the restore blocks
   dim3 grid((nrec + 3) / 4 + (a.rbps > 0 ? a.rbps * nshot : 0)), block(256);
 #define CASE(N)                                                                          \
