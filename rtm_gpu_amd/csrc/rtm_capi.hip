@@ -753,17 +753,23 @@ int rtm_create(const rtm_params* p, rtm_ctx** out) {
     rc = fail(c, RTM_E_NOMEM, "hipHostMalloc of the watch flag failed");
   }
   {
-    // the widest grid with the shortest runs the autotuner tries has the most records
     c->fseg_tb = g.n2;
     c->fseg_side = frame_segment_rows(g.nfd);
-    // (a split frame of a narrower shot may take a few records more than
-    // its FRAME_ALL list: at most two end records and one float4 record per
-    // band run beyond it)
-    const int min_tb = std::max(1, (g.n2 + 2) / 3);
-    std::vector<int4> tiles;
-    build_tiles(g, g.nx, 0, min_tb, std::max(1, c->fseg_side / 4), FRAME_ALL, tiles);
-    const int runs = 2 * ((g.n2 + min_tb - 1) / min_tb);
-    c->tile_cap = (2 * (int)tiles.size() + runs * (2 * ((g.n2 + 62) / 62) + 1)) * p->max_shots;
+    // the most records one shot of any width takes at the shortest runs the
+    // autotuner tries, over the three lists (a narrower shot may take more
+    // than the widest: its side bands may not pack into one wave)
+    const int min_tb = std::max(1, (g.n2 + 2) / 3), min_side = std::max(1, c->fseg_side / 4);
+    int most = 0;
+    for (int nx = 1; nx <= g.nx; ++nx) {
+      int tot = 0;
+      for (int kind = FRAME_ALL; kind <= FRAME_VEC; ++kind) {
+        int n = 0;
+        frame_records(g, nx, 0, min_tb, min_side, kind, nullptr, &n);
+        tot += n;
+      }
+      most = std::max(most, tot);
+    }
+    c->tile_cap = most * p->max_shots;
   }
   if (rc == RTM_OK) rc = dalloc_t(c, &c->d_tiles, (size_t)c->tile_cap);
   if (rc != RTM_OK) {
