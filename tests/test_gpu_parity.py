@@ -293,15 +293,18 @@ def test_frame_runs_keep_results():
         hip_context(case, nshot=1).set_frame_runs(n2 + 1, 20)
 
 
-def test_frame_split_keeps_results():
+@pytest.mark.parametrize("nfdmax", [2, 4, 6])
+def test_frame_split_keeps_results(nfdmax):
     """The band edges of the forward and receiver frames in the four-column
     kernel (frame_split = 1; n2 = 12, several float4 records per band run)
     give the fields, strips and images of the one-column kernel bit for bit,
     with whole-band and shortened frame runs (the receiver frame launch also
-    restores the strips of the reconstructed field)."""
+    restores the strips of the reconstructed field).  The designs' longest
+    operators are 2, 4 and 5: row loops unrolled 12 and 24 times, column halos
+    of one and two float4s."""
     from rtm_gpu_amd import hip
-    case = build_case(name="split", mod_nz=60, mod_nx=560, n2=12, nfdmax=4, nt=200, nshot=3,
-                      depth_step=20.0, seed=7)
+    case = build_case(name=f"split{nfdmax}", mod_nz=60, mod_nx=560, n2=12, nfdmax=nfdmax,
+                      nfdmin=min(2, nfdmax), nt=200, nshot=3, depth_step=20.0, seed=7)
     seis = np.stack([synth_gather(case, s) for s in range(3)])
     cfg = case["cfg"]
     for runs in (None, ((cfg["n2"] + 2) // 3, 10)):
